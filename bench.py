@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-crossings/s of the photon-shooting phase on the 256^3
+clumpy medium, one MI355X.
+
+A step is one plain iteration of the method on the full workload: emission
+and transport of 1e8 packets through the 256^3 log-normal field (4^3
+subgrids of 64^3 cells, 8 point sources) followed by the FP64 ionization
+update of all 16.8M cells.  Inputs are resident in HBM before the timed
+region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--engine walk|tiles]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("cell-crossings/s at 256³ clumpy; wall time to converged x_n "
+          "(plain vs warm)")
+PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(ncell, nsub, sigma_ln=1.0):
+    from parallelcmacionize_amd import workloads as W
+    box = W.box_10pc()
+    n = W.lognormal_density(ncell, sigma_ln=sigma_ln, seed=2)
+    n = W.global_to_subgrid_major(n, nsub)
+    pos, L = W.random_sources(8, box, seed=3, lum_seed=4)
+    return box, n, pos, L
+
+
+def cpu_baseline(box, ncell, nsub, n, xn, pos, L, seed, iteration,
+                 target_s, nthreads):
+    """Reference CPU path (the reference's DensitySubGrid::interact,
+    PhotonBuffer, MemorySpace and Lock, compiled with its -O3 -ffast-math
+    flags, task-parallel over subgrids) on a bounded sample of the same
+    workload; crossings of the sample counted by an FP64 context (bit-exact
+    reference walk)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    import parallelcmacionize_amd as cmi
+    ref = O.load_ref(fast=True)
+    kind = "reference"
+    c64 = cmi.Context(box, [ncell] * 3, nsub, cmi.PREC_FP64)
+    c64.set_fields(n, xn)
+    c64.set_sources(pos, L)
+
+    def sample(S):
+        pk, ss = c64.emit(S, seed, iteration)
+        c64.reset_intensity()
+        _, _, cr, _, _ = c64.transport_packets(pk, ss)
+        return pk, ss, int(cr.sum())
+
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nc = np.array([ncell] * 3, np.int32)
+    ns = np.array(nsub, np.int32)
+    J = np.zeros_like(n)
+
+    def run(pk, ss):
+        S = len(pk)
+        if ref is not None:
+            pool = 2 * (S // 200) + 2 * int(np.prod(nsub)) * 27 + 4096
+            return ref.ref_transport_tasks(
+                O.ptr(b, O._dp), O.ptr(nc, O._ip), O.ptr(ns, O._ip),
+                O.ptr(n, O._dp), O.ptr(xn, O._dp), O.ptr(J, O._dp),
+                pk.ctypes.data, O.ptr(ss, O._u32p), S, nthreads, pool)
+        og = O.OracleGrid(O.load_oracle(), b, nc, ns)
+        og.n[:] = n
+        og.xn[:] = xn
+        t0 = time.perf_counter()
+        og.transport(pk.copy(), ss, nthreads=nthreads)
+        return time.perf_counter() - t0
+
+    if ref is None:
+        kind = "port"
+    S0 = 20000
+    pk, ss, cr0 = sample(S0)
+    t_small = run(pk, ss)
+    S = int(min(4_000_000, max(S0, S0 * target_s / max(t_small, 1e-3))))
+    pk, ss, cr = sample(S)
+    t = run(pk, ss)
+    c64.close()
+    return {"value": cr / t, "unit": "crossings/s", "cores": nthreads,
+            "kind": kind,
+            "sample": ("%d packets of the same workload (same field state, "
+                       "same sources, iteration %d), %d crossings in %.2f s "
+                       "on %d threads" % (S, iteration, cr, t, nthreads))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--engine", default="walk", choices=["walk", "tiles"])
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--packets", type=float, default=1e8)
+    ap.add_argument("--ncell", type=int, default=256)
+    ap.add_argument("--nsub", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU "
+              "path (see DESIGN.md for how the transport would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import parallelcmacionize_amd as cmi
+    from parallelcmacionize_amd import workloads as W
+
+    N = int(args.packets)
+    ncell = args.ncell
+    nsub = [args.nsub] * 3
+    log("generating %d^3 log-normal field" % ncell)
+    box, n, pos, L = make_workload(ncell, nsub)
+    prec = cmi.PREC_FP32 if args.prec == "fp32" else cmi.PREC_FP64
+    ctx = cmi.Context(box, [ncell] * 3, nsub, prec)
+    ctx.set_engine(cmi.ENGINE_WALK if args.engine == "walk"
+                   else cmi.ENGINE_TILES)
+    ctx.set_fields(n, np.full(n.size, W.XN0))
+    ctx.set_sources(pos, L)
+    Ltot = ctx.total_luminosity()
+    seed = 1
+    for it in range(args.warmup):
+        s = ctx.transport(N, seed, it)
+        l1 = ctx.update_ionization(Ltot, N, W.ALPHA_B, form=1)
+        log("warmup %d: %.3f s transport, %d crossings, L1 %.3g"
+            % (it, s["seconds"], s["crossings"], l1))
+    ctx.synchronize()
+    ctx.kernel_timing(reset=True)
+    cross = hops = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s = ctx.transport(N, seed, args.warmup + k)
+        l1 = ctx.update_ionization(Ltot, N, W.ALPHA_B, form=1)
+        cross += s["crossings"]
+        hops += s["hops"]
+        log("step %d: %.3f s transport, %d crossings, L1 %.3g"
+            % (k, s["seconds"], s["crossings"], l1))
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    avg_kernel, calls = ctx.kernel_timing()
+    b_cross = 20 if prec == cmi.PREC_FP32 else 32
+    per_launch = (cross * b_cross + hops * 64) / max(calls, 1)
+    achieved = per_launch / avg_kernel
+    out = {
+        "metric": METRIC,
+        "value": cross / dt,
+        "unit": "crossings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == cmi.PREC_FP32 else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("clumpy medium %d^3 cells (%d^3 subgrids of %d^3), "
+                         "log-normal n_H (sigma_ln 1.0, P(k)~k^-11/3, mean "
+                         "1e8 m^-3), 8 sources L in [1e48,1e49] s^-1, %.0e "
+                         "packets per iteration; step = transport + FP64 "
+                         "ionization update" % (ncell, nsub[0],
+                                                ncell // nsub[0], N)),
+            "engine": args.engine,
+            "traversal": args.prec,
+            "deposits": "f64 atomics",
+            "cells": ncell ** 3,
+            "packets_per_step": N,
+            "crossings_per_step": cross / args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM,
+            "traffic": None,
+            "kernel": "walk_emit_kernel" if args.engine == "walk"
+                      else "tiles kernels",
+            "bytes_model": "%d B/crossing + 64 B/subgrid visit" % b_cross,
+            "avg_kernel_s": avg_kernel,
+        },
+    }
+    if not args.no_cpu:
+        xn_now = ctx.get_fields()[1]
+        nthreads = min(16, os.cpu_count() or 1)
+        log("cpu baseline on %d threads" % nthreads)
+        out["cpu_baseline"] = cpu_baseline(
+            box, ncell, nsub, n, xn_now, pos, L, seed,
+            args.warmup + args.steps, args.cpu_seconds, nthreads)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,211 @@
+/*
+ * cmi.h -- C ABI of the MI355X photon-packet transport and ionization engine.
+ *
+ * Drop-in for the hot path of ParallelCMacIonize: DensitySubGrid cell walking
+ * with optical-depth deposition into the intensity integrals, PhotonBuffer /
+ * TravelDirections routing of packets between subgrids, and the per-cell
+ * ionization-equilibrium update.  The reference exposes this path as inline
+ * C++ class methods; each entry point below names the method(s) it replaces.
+ *
+ * Conventions
+ *  - Every call returns CMI_OK (0) or a negative CMI_ERR_* code; nothing
+ *    aborts and no C++ exception crosses the ABI.  cmi_last_error() gives the
+ *    message of the last failure on a context.
+ *  - A context owns all device memory and one HIP stream on one GPU and is
+ *    used from one host thread at a time.  Several contexts may coexist.
+ *  - Host arrays are caller-owned and copied in and out.  Cell arrays use the
+ *    reference layout: subgrid-major (subgrid index ix*ny*nz + iy*nz + iz of
+ *    testDensitySubGrid_omp.cpp:1818-1822), and inside a subgrid the cell
+ *    order ix*ny*nz + iy*nz + iz (DensitySubGrid.hpp:207-210).
+ *  - TravelDirection values are those of TravelDirections.hpp:46-75.
+ */
+#ifndef CMI_H
+#define CMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define CMI_OK 0
+#define CMI_ERR_ARG (-1)       /* invalid argument */
+#define CMI_ERR_HIP (-2)       /* HIP runtime error */
+#define CMI_ERR_ALLOC (-3)     /* device or host allocation failed */
+#define CMI_ERR_OVERFLOW (-4)  /* packet pool / bin capacity exceeded */
+#define CMI_ERR_DIRECTION (-5) /* invalid exit mask or walk bound exceeded */
+#define CMI_ERR_STATE (-6)     /* call out of order (e.g. no sources set) */
+#define CMI_ERR_NODEVICE (-7)  /* no usable GPU */
+
+/* ---- arithmetic modes ---- */
+/* FP64 throughout, reference arithmetic: bit-identical walks to
+ * DensitySubGrid::interact. */
+#define CMI_PREC_FP64 0
+/* FP32 geometry and optical depth (cell units, kappa = sigma n x_n stored as
+ * FP32), FP64 deposits into J. */
+#define CMI_PREC_FP32 1
+
+/* ---- transport engines ---- */
+#define CMI_ENGINE_WALK 0  /* one persistent thread per packet, global J */
+#define CMI_ENGINE_TILES 1 /* packets binned per LDS tile, rounds of hops */
+
+/* TravelDirection values (TravelDirections.hpp:46-75) used by the ABI. */
+#define CMI_TRAVELDIRECTION_INSIDE 0
+#define CMI_TRAVELDIRECTION_NUMBER 27
+#define CMI_NEIGHBOUR_OUTSIDE 0xffffffffu /* DensitySubGrid.hpp:53 */
+
+/* Photon packet (Photon.hpp:52-65, without the packet type). 72 bytes. */
+typedef struct {
+  double pos[3];   /* position (m) */
+  double dir[3];   /* unit direction */
+  double tau;      /* remaining target optical depth */
+  double sigma;    /* photoionization cross section (m^2) */
+  double weight;   /* packet weight */
+} cmi_photon;
+
+/* Per-transport statistics (the reference keeps no such counters). */
+typedef struct {
+  uint64_t packets;   /* packets transported */
+  uint64_t crossings; /* cell crossings = deposits into J */
+  uint64_t hops;      /* subgrid (walk) or tile (tiles) visits */
+  uint64_t absorbed;  /* packets that reached their target optical depth */
+  uint64_t escaped;   /* packets that left the box */
+  uint64_t errors;    /* packets stopped on an invalid exit or a walk bound */
+  uint32_t rounds;    /* hop rounds (tiles engine), 0 for walk */
+  uint32_t reserved;
+  double seconds;     /* device time of the transport (HIP events) */
+} cmi_stats;
+
+/* Tuning of the tiles engine (0 = default for any field). */
+typedef struct {
+  int tile[3];           /* LDS tile in cells (powers of two dividing the
+                            subgrid) */
+  int block_threads;     /* threads per workgroup */
+  int packets_per_block; /* packets one workgroup visit processes */
+  int walk_blocks;       /* persistent grid of the walk engine */
+} cmi_tuning;
+
+typedef struct cmi_ctx cmi_ctx;
+
+/* Create a grid of nsub[0]*nsub[1]*nsub[2] subgrids covering box (anchor
+ * box[0..2], sides box[3..5], m) with ncell cells in total per axis; fields
+ * start at n = 1e8 m^-3, x_n = 1e-6, J = 0 (DensitySubGrid.hpp:544-551).
+ * Replaces the subgrid set-up of testDensitySubGrid_omp.cpp:1805-1877
+ * (DensitySubGrid::DensitySubGrid, DensitySubGrid.hpp:516, and
+ * set_neighbour, DensitySubGrid.hpp:831). */
+int cmi_create(const double box[6], const int ncell[3], const int nsub[3],
+               int precision, cmi_ctx **out);
+
+/* Free everything the context owns (DensitySubGrid::~DensitySubGrid,
+ * DensitySubGrid.hpp:758). */
+void cmi_destroy(cmi_ctx *ctx);
+
+/* Message of the last failed call on ctx (never NULL). */
+const char *cmi_last_error(const cmi_ctx *ctx);
+
+/* Select the transport engine and its tuning (tuning may be NULL). */
+int cmi_set_engine(cmi_ctx *ctx, int engine, const cmi_tuning *tuning);
+
+/* Copy number density and neutral fraction in (either may be NULL to keep
+ * the current values).  Replaces DensitySubGrid::set_number_density and
+ * set_neutral_fraction (DensitySubGrid.hpp:1116-1131). */
+int cmi_set_fields(cmi_ctx *ctx, const double *nH, const double *xn);
+
+/* Copy intensity integrals in (DensitySubGrid::set_intensity_integral,
+ * DensitySubGrid.hpp:1139).  Values are in the reference's units (m^3). */
+int cmi_set_intensity(cmi_ctx *ctx, const double *J);
+
+/* Copy intensity integrals (m^3) and neutral fractions out (either may be
+ * NULL).  Replaces get_intensity_integral and get_neutral_fraction
+ * (DensitySubGrid.hpp:1614, 790). */
+int cmi_get_fields(cmi_ctx *ctx, double *J, double *xn);
+
+/* Zero the intensity integrals (the J reset of
+ * DensitySubGrid::compute_neutral_fraction, DensitySubGrid.hpp:1656). */
+int cmi_reset_intensity(cmi_ctx *ctx);
+
+/* DensitySubGrid::interact (DensitySubGrid.hpp:1180-1323) for n packets that
+ * all enter subgrid `subgrid` through TravelDirection entry_dir: packets are
+ * updated in place (position, remaining tau), exit_dir receives the returned
+ * TravelDirection (0 = absorbed) and crossings the number of cells crossed.
+ * Deposits are added to J.  FP64 contexts only. */
+int cmi_interact_batch(cmi_ctx *ctx, uint32_t subgrid, int entry_dir,
+                       cmi_photon *p, size_t n, int32_t *exit_dir,
+                       uint32_t *crossings);
+
+/* Whole-grid transport of an explicit packet list: each packet starts INSIDE
+ * start_sub[i] and is routed from subgrid to subgrid (the loop of
+ * testDensitySubGrid_omp.cpp:607-666 and 1149-1329 with
+ * TravelDirections::output_to_input_direction, TravelDirections.hpp:89)
+ * until absorbed (status 0) or out of the box (status 1); negative status =
+ * invalid exit or walk bound.  Deposits are added to J.  Any of status,
+ * crossings, hops may be NULL. */
+int cmi_transport_packets(cmi_ctx *ctx, cmi_photon *p,
+                          const uint32_t *start_sub, size_t n, int32_t *status,
+                          uint32_t *crossings, uint32_t *hops,
+                          cmi_stats *stats);
+
+/* Point sources (Source.hpp:57): positions (3 per source, m) and ionizing
+ * luminosities (s^-1).  Up to 64 sources. */
+int cmi_set_sources(cmi_ctx *ctx, const double *pos, const double *L,
+                    int nsrc);
+
+/* Packets the transport draws for (npackets, seed, iteration), numbers
+ * first .. first+count-1: isotropic directions, tau = -ln u, the given cross
+ * section, weight 1 (fill_buffer, testDensitySubGrid_omp.cpp:556-597, with a
+ * Philox4x32-10 counter-based generator instead of RANLUX).  start_sub may be
+ * NULL. */
+int cmi_emit(cmi_ctx *ctx, uint64_t npackets, uint64_t first, uint64_t count,
+             uint64_t seed, uint32_t iteration, double sigma, cmi_photon *out,
+             uint32_t *start_sub);
+
+/* One photon-shooting phase: emit npackets over the sources (split as
+ * testDensitySubGrid_omp.cpp:2199-2208 / 2240-2273) and transport them all,
+ * depositing into J (the worker loop of testDensitySubGrid_omp.cpp:2274-2401).
+ */
+int cmi_transport(cmi_ctx *ctx, uint64_t npackets, uint64_t seed,
+                  uint32_t iteration, double sigma, cmi_stats *stats);
+
+/* DensitySubGrid::compute_neutral_fraction (DensitySubGrid.hpp:1628-1658) on
+ * every cell, with the recombination rate alphaH a parameter (the reference
+ * fixes 4e-19).  form 0 = the reference expression, form 1 = the
+ * cancellation-free root.  J is reset.  l1_change (may be NULL) receives the
+ * mean |x_n(new) - x_n(old)| over all cells. */
+int cmi_update_ionization(cmi_ctx *ctx, double luminosity, uint64_t npackets,
+                          double alphaH, int form, double *l1_change);
+
+/* Total luminosity of the sources set by cmi_set_sources. */
+double cmi_total_luminosity(const cmi_ctx *ctx);
+
+/* TravelDirections::output_to_input_direction (TravelDirections.hpp:89) and
+ * TravelDirections::get_output_direction(mask) (TravelDirections.hpp:315):
+ * host-side, no device needed; -1 on invalid input. */
+int cmi_output_to_input_direction(int output_direction);
+int cmi_direction_from_mask(int mask);
+
+/* Neighbour table of subgrid `subgrid`: 27 entries indexed by
+ * TravelDirection, CMI_NEIGHBOUR_OUTSIDE where none
+ * (DensitySubGrid::get_neighbour, DensitySubGrid.hpp:820). */
+int cmi_get_neighbours(const cmi_ctx *ctx, uint32_t subgrid,
+                       uint32_t ngbs[27]);
+
+/* Number of subgrids and cells (DensitySubGrid::get_number_of_cells,
+ * DensitySubGrid.hpp:780, summed over subgrids). */
+int cmi_get_sizes(const cmi_ctx *ctx, uint64_t *nsubgrid, uint64_t *ncell);
+
+/* Device timing of the last kernel sequence on the context's stream, for
+ * the roofline figure: average duration (s) of the dominant transport
+ * kernel over the calls since the last reset, and the number of calls. */
+int cmi_kernel_timing(cmi_ctx *ctx, double *avg_seconds, uint64_t *calls,
+                      int reset);
+
+/* Synchronise the context's stream. */
+int cmi_synchronize(cmi_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMI_H */

@@ -1,0 +1,288 @@
+"""MI355X photon-packet transport and ionization engine (ctypes front end).
+
+The product is the C-ABI library ``libcmi.so`` (include/cmi.h) built from the
+hand-written gfx950 HIP kernels in ``csrc/``.  This module loads it and wraps
+each entry point; it has no fallback: if the library is missing or no GPU is
+present, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "CMIError", "Context", "PHOTON_DTYPE", "load_library", "PREC_FP64",
+    "PREC_FP32", "ENGINE_WALK", "ENGINE_TILES", "INSIDE", "output_to_input",
+    "direction_from_mask", "STATS_FIELDS",
+]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcmi.so")
+
+PREC_FP64 = 0
+PREC_FP32 = 1
+ENGINE_WALK = 0
+ENGINE_TILES = 1
+INSIDE = 0
+NEIGHBOUR_OUTSIDE = 0xFFFFFFFF
+
+PHOTON_DTYPE = np.dtype([("pos", "<f8", 3), ("dir", "<f8", 3), ("tau", "<f8"),
+                         ("sigma", "<f8"), ("weight", "<f8")])
+
+_ERRORS = {
+    -1: "CMI_ERR_ARG", -2: "CMI_ERR_HIP", -3: "CMI_ERR_ALLOC",
+    -4: "CMI_ERR_OVERFLOW", -5: "CMI_ERR_DIRECTION", -6: "CMI_ERR_STATE",
+    -7: "CMI_ERR_NODEVICE",
+}
+
+
+class CMIError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (_ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class Stats(C.Structure):
+    _fields_ = [("packets", C.c_uint64), ("crossings", C.c_uint64),
+                ("hops", C.c_uint64), ("absorbed", C.c_uint64),
+                ("escaped", C.c_uint64), ("errors", C.c_uint64),
+                ("rounds", C.c_uint32), ("reserved", C.c_uint32),
+                ("seconds", C.c_double)]
+
+
+STATS_FIELDS = [f for f, _ in Stats._fields_ if f != "reserved"]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile", C.c_int * 3), ("block_threads", C.c_int),
+                ("packets_per_block", C.c_int), ("walk_blocks", C.c_int)]
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+_vp = C.c_void_p
+
+# entry point -> (restype, argtypes); every symbol include/cmi.h declares
+SIGNATURES = {
+    "cmi_create": (C.c_int, [_dp, _ip, _ip, C.c_int, C.POINTER(_vp)]),
+    "cmi_destroy": (None, [_vp]),
+    "cmi_last_error": (C.c_char_p, [_vp]),
+    "cmi_set_engine": (C.c_int, [_vp, C.c_int, C.POINTER(Tuning)]),
+    "cmi_set_fields": (C.c_int, [_vp, _dp, _dp]),
+    "cmi_set_intensity": (C.c_int, [_vp, _dp]),
+    "cmi_get_fields": (C.c_int, [_vp, _dp, _dp]),
+    "cmi_reset_intensity": (C.c_int, [_vp]),
+    "cmi_interact_batch": (C.c_int, [_vp, C.c_uint32, C.c_int, _vp, C.c_size_t,
+                                      _i32p, _u32p]),
+    "cmi_transport_packets": (C.c_int, [_vp, _vp, _u32p, C.c_size_t, _i32p,
+                                         _u32p, _u32p, C.POINTER(Stats)]),
+    "cmi_set_sources": (C.c_int, [_vp, _dp, _dp, C.c_int]),
+    "cmi_emit": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
+                           C.c_uint32, C.c_double, _vp, _u32p]),
+    "cmi_transport": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.c_uint32,
+                                C.c_double, C.POINTER(Stats)]),
+    "cmi_update_ionization": (C.c_int, [_vp, C.c_double, C.c_uint64, C.c_double,
+                                        C.c_int, _dp]),
+    "cmi_total_luminosity": (C.c_double, [_vp]),
+    "cmi_output_to_input_direction": (C.c_int, [C.c_int]),
+    "cmi_direction_from_mask": (C.c_int, [C.c_int]),
+    "cmi_get_neighbours": (C.c_int, [_vp, C.c_uint32, _u32p]),
+    "cmi_get_sizes": (C.c_int, [_vp, _u64p, _u64p]),
+    "cmi_kernel_timing": (C.c_int, [_vp, _dp, _u64p, C.c_int]),
+    "cmi_synchronize": (C.c_int, [_vp]),
+}
+
+
+def load_library(path=LIB_PATH):
+    """Load libcmi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise CMIError(-6, "libcmi.so not built: run "
+                           "python -m parallelcmacionize_amd.build")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def output_to_input(d):
+    """TravelDirections::output_to_input_direction (TravelDirections.hpp:89)."""
+    return load_library().cmi_output_to_input_direction(int(d))
+
+
+def direction_from_mask(mask):
+    """TravelDirections::get_output_direction(mask) (TravelDirections.hpp:315)."""
+    return load_library().cmi_direction_from_mask(int(mask))
+
+
+class Context:
+    """One grid of subgrids on one GPU (owner of a cmi_ctx)."""
+
+    def __init__(self, box, ncell, nsub, precision=PREC_FP64):
+        self.lib = load_library()
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        nc = np.ascontiguousarray(ncell, dtype=np.int32)
+        ns = np.ascontiguousarray(nsub, dtype=np.int32)
+        h = _vp()
+        rc = self.lib.cmi_create(_p(b, _dp), _p(nc, _ip), _p(ns, _ip),
+                                 int(precision), C.byref(h))
+        if rc != 0:
+            raise CMIError(rc, "cmi_create failed")
+        self.h = h
+        self.box = b
+        self.ncell = nc
+        self.nsub = ns
+        self.precision = precision
+        nsg = C.c_uint64()
+        ncl = C.c_uint64()
+        self.lib.cmi_get_sizes(h, C.byref(nsg), C.byref(ncl))
+        self.nsubgrid = nsg.value
+        self.ncell_total = ncl.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cmi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CMIError(rc, self.lib.cmi_last_error(self.h).decode())
+
+    def set_engine(self, engine, tile=None, block_threads=0,
+                   packets_per_block=0, walk_blocks=0):
+        t = Tuning()
+        if tile is not None:
+            for a in range(3):
+                t.tile[a] = int(tile[a])
+        t.block_threads = int(block_threads)
+        t.packets_per_block = int(packets_per_block)
+        t.walk_blocks = int(walk_blocks)
+        self._check(self.lib.cmi_set_engine(self.h, int(engine), C.byref(t)))
+
+    def set_fields(self, nH=None, xn=None):
+        a = None if nH is None else np.ascontiguousarray(nH, dtype=np.float64)
+        b = None if xn is None else np.ascontiguousarray(xn, dtype=np.float64)
+        for v in (a, b):
+            if v is not None and v.size != self.ncell_total:
+                raise ValueError("field size %d != %d" % (v.size, self.ncell_total))
+        self._check(self.lib.cmi_set_fields(
+            self.h, None if a is None else _p(a, _dp),
+            None if b is None else _p(b, _dp)))
+
+    def set_intensity(self, J):
+        a = np.ascontiguousarray(J, dtype=np.float64)
+        if a.size != self.ncell_total:
+            raise ValueError("field size")
+        self._check(self.lib.cmi_set_intensity(self.h, _p(a, _dp)))
+
+    def reset_intensity(self):
+        self._check(self.lib.cmi_reset_intensity(self.h))
+
+    def get_fields(self):
+        J = np.zeros(self.ncell_total)
+        x = np.zeros(self.ncell_total)
+        self._check(self.lib.cmi_get_fields(self.h, _p(J, _dp), _p(x, _dp)))
+        return J, x
+
+    def interact_batch(self, subgrid, entry_dir, photons):
+        p = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE)
+        n = len(p)
+        ex = np.zeros(n, np.int32)
+        cr = np.zeros(n, np.uint32)
+        self._check(self.lib.cmi_interact_batch(self.h, int(subgrid),
+                                                int(entry_dir), p.ctypes.data,
+                                                n, _p(ex, _i32p), _p(cr, _u32p)))
+        return p, ex, cr
+
+    def transport_packets(self, photons, start_sub):
+        p = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE).copy()
+        ss = np.ascontiguousarray(start_sub, dtype=np.uint32)
+        n = len(p)
+        st = np.zeros(n, np.int32)
+        cr = np.zeros(n, np.uint32)
+        hp = np.zeros(n, np.uint32)
+        s = Stats()
+        self._check(self.lib.cmi_transport_packets(
+            self.h, p.ctypes.data, _p(ss, _u32p), n, _p(st, _i32p),
+            _p(cr, _u32p), _p(hp, _u32p), C.byref(s)))
+        return p, st, cr, hp, _stats_dict(s)
+
+    def set_sources(self, pos, L):
+        p = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        l = np.ascontiguousarray(L, dtype=np.float64).reshape(-1)
+        self._check(self.lib.cmi_set_sources(self.h, _p(p, _dp), _p(l, _dp),
+                                             len(l)))
+
+    def total_luminosity(self):
+        return self.lib.cmi_total_luminosity(self.h)
+
+    def emit(self, npackets, seed, iteration, sigma=6.3e-22, first=0,
+             count=None):
+        if count is None:
+            count = npackets - first
+        out = np.zeros(count, PHOTON_DTYPE)
+        ss = np.zeros(count, np.uint32)
+        self._check(self.lib.cmi_emit(self.h, npackets, first, count, seed,
+                                      iteration, sigma, out.ctypes.data,
+                                      _p(ss, _u32p)))
+        return out, ss
+
+    def transport(self, npackets, seed, iteration, sigma=6.3e-22):
+        s = Stats()
+        self._check(self.lib.cmi_transport(self.h, int(npackets), int(seed),
+                                           int(iteration), float(sigma),
+                                           C.byref(s)))
+        return _stats_dict(s)
+
+    def update_ionization(self, luminosity, npackets, alpha, form=0):
+        l1 = C.c_double()
+        self._check(self.lib.cmi_update_ionization(self.h, float(luminosity),
+                                                   int(npackets), float(alpha),
+                                                   int(form), C.byref(l1)))
+        return l1.value
+
+    def neighbours(self, subgrid):
+        out = np.zeros(27, np.uint32)
+        self._check(self.lib.cmi_get_neighbours(self.h, int(subgrid),
+                                                _p(out, _u32p)))
+        return out
+
+    def kernel_timing(self, reset=False):
+        avg = C.c_double()
+        calls = C.c_uint64()
+        self._check(self.lib.cmi_kernel_timing(self.h, C.byref(avg),
+                                               C.byref(calls), int(reset)))
+        return avg.value, calls.value
+
+    def synchronize(self):
+        self._check(self.lib.cmi_synchronize(self.h))
+
+
+def _stats_dict(s):
+    return {f: getattr(s, f) for f in STATS_FIELDS}

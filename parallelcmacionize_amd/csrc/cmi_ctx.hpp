@@ -1,0 +1,71 @@
+// cmi_ctx.hpp -- the context behind the opaque cmi_ctx handle of cmi.h,
+// shared by the C-ABI layer and the engines' host code.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/cmi.h"
+#include "cmi_internal.hpp"
+
+namespace cmi {
+struct TilesEngine;
+}
+
+struct cmi_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  int precision = CMI_PREC_FP64;
+  int engine = CMI_ENGINE_WALK;
+  cmi_tuning tuning{};
+  double box[6];
+  int ncell[3], nsub[3], sc[3], tile[3];
+  size_t nsubgrid = 0, cps = 0, ncell_total = 0;
+  std::vector<double> anchors;   // [nsubgrid][3]
+  std::vector<uint32_t> ngbs;    // [nsubgrid][27]
+  cmi::GridDev g{};
+  cmi::FieldsDev f{};
+  double *d_anchors = nullptr;
+  uint32_t *d_ngbs = nullptr;
+  cmi::Counters *d_cnt = nullptr;
+  cmi::Counters *h_cnt = nullptr;  // pinned
+  cmi::SourcesDev *d_src = nullptr;
+  cmi::SourcesDev *h_src = nullptr;  // pinned
+  double *d_partial = nullptr;
+  std::vector<double> h_partial;
+  std::vector<double> src_pos, src_L;
+  double sigma = 6.3e-22;  // cross section folded into kappa (FP32 mode)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kernel_time_sum = 0.;
+  uint64_t kernel_calls = 0;
+  cmi::TilesEngine *tiles = nullptr;
+  std::string err;
+};
+
+namespace cmi {
+
+inline int fail(cmi_ctx *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define CMI_TRY_HIP(ctx, call)                                              \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      return ::cmi::fail(ctx, CMI_ERR_HIP,                                  \
+                         std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                       \
+  } while (0)
+
+// Tiles engine (cmi_tiles.hip).
+int tiles_transport_packets(cmi_ctx *c, cmi_photon *p,
+                            const uint32_t *start_sub, size_t n,
+                            int32_t *status, uint32_t *crossings,
+                            uint32_t *hops, cmi_stats *stats);
+int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
+                    uint32_t iteration, double sigma, cmi_stats *stats);
+void tiles_destroy(cmi_ctx *c);
+
+}  // namespace cmi

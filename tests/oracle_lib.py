@@ -1,0 +1,222 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the
+reference-header harness (oracle/_ref/libref.so).
+
+Test infrastructure: imported by tests, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg only.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+REF_FAST_SO = os.path.join(ROOT, "oracle", "_ref", "libref_fast.so")
+
+PHOTON_DTYPE = np.dtype([("pos", "<f8", 3), ("dir", "<f8", 3), ("tau", "<f8"),
+                         ("sigma", "<f8"), ("weight", "<f8")])
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u32p = C.POINTER(C.c_uint32)
+_i32p = C.POINTER(C.c_int32)
+_u64p = C.POINTER(C.c_uint64)
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class OrcSubgrid(C.Structure):
+    _fields_ = [("anchor", C.c_double * 3), ("cell_size", C.c_double * 3),
+                ("inv_cell_size", C.c_double * 3), ("ncell", C.c_int * 4),
+                ("ngbs", C.c_uint32 * 27), ("n", _dp), ("xn", _dp),
+                ("J", _dp)]
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("box", C.c_double * 6), ("ncell", C.c_int * 3),
+                ("nsub", C.c_int * 3), ("nsubgrid", C.c_int),
+                ("cells_per_subgrid", C.c_size_t),
+                ("ncell_total", C.c_size_t),
+                ("sub", C.POINTER(OrcSubgrid)), ("n", _dp), ("xn", _dp),
+                ("J", _dp)]
+
+
+def load_oracle():
+    lib = C.CDLL(ORACLE_SO)
+    lib.orc_output_to_input_direction.argtypes = [C.c_int]
+    lib.orc_direction_offset.argtypes = [C.c_int, _ip]
+    lib.orc_direction_from_offset.argtypes = [_ip]
+    lib.orc_direction_from_mask.argtypes = [C.c_int]
+    lib.orc_get_output_direction.argtypes = [C.POINTER(OrcSubgrid), _ip]
+    lib.orc_get_start_index.argtypes = [C.POINTER(OrcSubgrid), _dp, C.c_int,
+                                        _ip]
+    lib.orc_subgrid_init.argtypes = [C.POINTER(OrcSubgrid), _dp, _ip]
+    lib.orc_interact.argtypes = [C.POINTER(OrcSubgrid), C.c_void_p, C.c_int,
+                                 _u32p]
+    lib.orc_is_compatible_output_direction.argtypes = [_dp, C.c_int]
+    lib.orc_is_compatible_input_direction.argtypes = [_dp, C.c_int]
+    lib.orc_grid_create.argtypes = [C.POINTER(OrcGrid), _dp, _ip, _ip]
+    lib.orc_grid_destroy.argtypes = [C.POINTER(OrcGrid)]
+    lib.orc_grid_source_subgrid.argtypes = [C.POINTER(OrcGrid), _dp]
+    lib.orc_grid_source_subgrid.restype = C.c_uint32
+    lib.orc_transport_list.argtypes = [C.POINTER(OrcGrid), C.c_void_p, _u32p,
+                                       C.c_size_t, _i32p, _u32p, _u32p,
+                                       C.c_int]
+    lib.orc_compute_neutral_fraction.argtypes = [C.POINTER(OrcGrid), C.c_double,
+                                                 C.c_double, C.c_double,
+                                                 C.c_int]
+    lib.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+    lib.orc_uniform_open.argtypes = [C.c_uint32, C.c_uint32]
+    lib.orc_uniform_open.restype = C.c_double
+    lib.orc_source_counts.argtypes = [_dp, C.c_int, C.c_uint64, C.c_uint64,
+                                      C.c_uint32, _u64p]
+    lib.orc_emit.argtypes = [C.POINTER(OrcGrid), _dp, _dp, C.c_int, C.c_uint64,
+                             C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32,
+                             C.c_double, C.c_void_p, _u32p]
+    lib.orc_iteration.argtypes = [C.POINTER(OrcGrid), _dp, _dp, C.c_int,
+                                  C.c_uint64, C.c_uint64, C.c_uint32,
+                                  C.c_double, C.c_double, C.c_int, C.c_int,
+                                  _dp]
+    lib.orc_iteration.restype = C.c_uint64
+    return lib
+
+
+def load_ref(fast=False):
+    path = REF_FAST_SO if fast else REF_SO
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.ref_get_output_direction.argtypes = [_ip, _ip]
+    lib.ref_direction_from_mask.argtypes = [C.c_int]
+    lib.ref_output_to_input_direction.argtypes = [C.c_int]
+    lib.ref_is_compatible_output_direction.argtypes = [_dp, C.c_int]
+    lib.ref_is_compatible_input_direction.argtypes = [_dp, C.c_int]
+    lib.ref_get_start_index.argtypes = [_dp, _ip, _dp, C.c_int, _ip]
+    lib.ref_interact_list.argtypes = [_dp, _ip, _dp, _dp, _dp, C.c_void_p,
+                                      C.c_size_t, C.c_int, _i32p, _u32p]
+    lib.ref_transport_list.argtypes = [_dp, _ip, _ip, _dp, _dp, _dp,
+                                       C.c_void_p, _u32p, C.c_size_t, _i32p,
+                                       _u32p, _u32p]
+    lib.ref_compute_neutral_fraction.argtypes = [_dp, _ip, _dp, _dp, C.c_double,
+                                                 C.c_uint, _dp]
+    lib.ref_transport_tasks.argtypes = [_dp, _ip, _ip, _dp, _dp, _dp,
+                                        C.c_void_p, _u32p, C.c_size_t, C.c_int,
+                                        C.c_size_t]
+    lib.ref_transport_tasks.restype = C.c_double
+    return lib
+
+
+class OracleGrid:
+    """Owner of an orc_grid (subgrid-major field arrays exposed as numpy)."""
+
+    def __init__(self, lib, box, ncell, nsub):
+        self.lib = lib
+        self.g = OrcGrid()
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        nc = np.ascontiguousarray(ncell, dtype=np.int32)
+        ns = np.ascontiguousarray(nsub, dtype=np.int32)
+        rc = lib.orc_grid_create(C.byref(self.g), ptr(b, _dp), ptr(nc, _ip),
+                                 ptr(ns, _ip))
+        if rc != 0:
+            raise ValueError("orc_grid_create failed: %d" % rc)
+        n = self.g.ncell_total
+        self.n = np.ctypeslib.as_array(self.g.n, shape=(n,))
+        self.xn = np.ctypeslib.as_array(self.g.xn, shape=(n,))
+        self.J = np.ctypeslib.as_array(self.g.J, shape=(n,))
+        self.box = b
+        self.ncell = nc
+        self.nsub = ns
+
+    def __del__(self):
+        try:
+            self.lib.orc_grid_destroy(C.byref(self.g))
+        except Exception:
+            pass
+
+    def source_subgrid(self, pos):
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        return self.lib.orc_grid_source_subgrid(C.byref(self.g), ptr(p, _dp))
+
+    def transport(self, photons, start_sub, nthreads=1):
+        n = len(photons)
+        st = np.zeros(n, np.int32)
+        cr = np.zeros(n, np.uint32)
+        hp = np.zeros(n, np.uint32)
+        ss = np.ascontiguousarray(start_sub, dtype=np.uint32)
+        self.lib.orc_transport_list(C.byref(self.g), photons.ctypes.data,
+                                    ptr(ss, _u32p), n, ptr(st, _i32p),
+                                    ptr(cr, _u32p), ptr(hp, _u32p), nthreads)
+        return st, cr, hp
+
+    def compute_neutral_fraction(self, L, N, alpha, form=0):
+        self.lib.orc_compute_neutral_fraction(C.byref(self.g), L, float(N),
+                                              alpha, form)
+
+    def emit(self, src_pos, src_L, npackets, seed, iteration, sigma=6.3e-22,
+             first=0, count=None):
+        if count is None:
+            count = npackets - first
+        sp = np.ascontiguousarray(src_pos, dtype=np.float64).reshape(-1)
+        sl = np.ascontiguousarray(src_L, dtype=np.float64)
+        out = np.zeros(count, PHOTON_DTYPE)
+        ss = np.zeros(count, np.uint32)
+        self.lib.orc_emit(C.byref(self.g), ptr(sp, _dp), ptr(sl, _dp), len(sl),
+                          npackets, first, count, seed, iteration, sigma,
+                          out.ctypes.data, ptr(ss, _u32p))
+        return out, ss
+
+    def iteration(self, src_pos, src_L, npackets, seed, iteration,
+                  sigma=6.3e-22, alpha=2.7e-19, form=0, nthreads=1):
+        sp = np.ascontiguousarray(src_pos, dtype=np.float64).reshape(-1)
+        sl = np.ascontiguousarray(src_L, dtype=np.float64)
+        l1 = C.c_double(0.0)
+        nc = self.lib.orc_iteration(C.byref(self.g), ptr(sp, _dp),
+                                    ptr(sl, _dp), len(sl), npackets, seed,
+                                    iteration, sigma, alpha, form, nthreads,
+                                    C.byref(l1))
+        return nc, l1.value
+
+
+def make_subgrid(lib, box, ncell):
+    s = OrcSubgrid()
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    lib.orc_subgrid_init(C.byref(s), ptr(b, _dp), ptr(nc, _ip))
+    return s
+
+
+def ref_transport_list(ref, box, ncell, nsub, n, xn, photons, start_sub):
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    ns = np.ascontiguousarray(nsub, dtype=np.int32)
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    xx = np.ascontiguousarray(xn, dtype=np.float64)
+    J = np.zeros_like(nn)
+    m = len(photons)
+    st = np.zeros(m, np.int32)
+    cr = np.zeros(m, np.uint32)
+    hp = np.zeros(m, np.uint32)
+    ss = np.ascontiguousarray(start_sub, dtype=np.uint32)
+    ref.ref_transport_list(ptr(b, _dp), ptr(nc, _ip), ptr(ns, _ip),
+                           ptr(nn, _dp), ptr(xx, _dp), ptr(J, _dp),
+                           photons.ctypes.data, ptr(ss, _u32p), m,
+                           ptr(st, _i32p), ptr(cr, _u32p), ptr(hp, _u32p))
+    return J, st, cr, hp
+
+
+def ref_interact_list(ref, box, ncell, n, xn, photons, input_direction):
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    xx = np.ascontiguousarray(xn, dtype=np.float64)
+    J = np.zeros_like(nn)
+    m = len(photons)
+    ex = np.zeros(m, np.int32)
+    cr = np.zeros(m, np.uint32)
+    ref.ref_interact_list(ptr(b, _dp), ptr(nc, _ip), ptr(nn, _dp),
+                          ptr(xx, _dp), ptr(J, _dp), photons.ctypes.data, m,
+                          input_direction, ptr(ex, _i32p), ptr(cr, _u32p))
+    return J, ex, cr

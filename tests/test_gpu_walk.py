@@ -1,0 +1,236 @@
+"""GPU parity of the walk engine and the ionization kernel against the oracle
+(through the C ABI).  Bars: crossing counts, exit directions, positions and
+remaining optical depth bit-exact in FP64; per-cell intensity sums within
+1e-6 relative (the GPU sums with atomics in a different order); the ionization
+update bit-exact for the reference form."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import parallelcmacionize_amd as cmi
+from parallelcmacionize_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PC = W.PC
+J_RTOL = 1e-6
+
+
+def _close_J(a, b, rtol=J_RTOL):
+    scale = np.abs(b).max()
+    assert np.all(np.abs(a - b) <= rtol * np.maximum(np.abs(b), 1e-300) +
+                  1e-14 * scale), np.abs(a - b).max()
+
+
+def _bitwise_equal(a, b):
+    return (np.ascontiguousarray(a).view(np.uint8) ==
+            np.ascontiguousarray(b).view(np.uint8)).all()
+
+
+def test_interact_batch_matches_golden(golden):
+    g = golden("interact_16.npz")
+    with cmi.Context(g["box"], g["ncell"], [1, 1, 1], cmi.PREC_FP64) as ctx:
+        ctx.set_fields(g["n"], g["xn"])
+        for k in range(3):
+            print("interact golden case", k, flush=True)
+            ctx.reset_intensity()
+            p, ex, cr = ctx.interact_batch(0, int(g["in_dir_%d" % k]),
+                                           g["photons_in_%d" % k])
+            assert (ex == g["exit_%d" % k]).all()
+            assert (cr == g["crossings_%d" % k]).all()
+            assert _bitwise_equal(p, g["photons_out_%d" % k])
+            J, _ = ctx.get_fields()
+            _close_J(J, g["J_%d" % k])
+
+
+@pytest.mark.parametrize("in_dir", [0, 1, 8, 11, 14, 19, 21, 26])
+def test_interact_batch_matches_oracle_anisotropic(oracle, in_dir):
+    rng = np.random.default_rng(100 + in_dir)
+    box = np.array([1.0 * PC, -2.0 * PC, 0.5 * PC, 1.1 * PC, 0.9 * PC, 1.3 * PC])
+    ncell = np.array([8, 16, 4], np.int32)
+    nc = int(np.prod(ncell))
+    n = 1e8 * np.exp(rng.normal(0, 1.5, nc))
+    xn = 10 ** rng.uniform(-7, 0, nc)
+    m = 2000
+    p = np.zeros(m, O.PHOTON_DTYPE)
+    p["pos"] = box[:3] + rng.uniform(0, 1, (m, 3)) * box[3:]
+    d = rng.normal(size=(m, 3))
+    p["dir"] = d / np.linalg.norm(d, axis=1)[:, None]
+    off = np.zeros(3, np.int32)
+    oracle.orc_direction_offset(in_dir, O.ptr(off, O._ip))
+    for a in range(3):
+        if off[a]:
+            p["pos"][:, a] = box[a] + (box[3 + a] if off[a] > 0 else 0.)
+            p["dir"][:, a] = -off[a] * np.abs(p["dir"][:, a])
+    p["tau"] = rng.exponential(2.0, m)
+    p["sigma"] = 6.3e-22
+    p["weight"] = rng.uniform(0.5, 2.0, m)
+    from test_oracle import _interact_oracle
+    po, exo, cro, Jo = _interact_oracle(oracle, box, ncell, n, xn, p, in_dir)
+    with cmi.Context(box, ncell, [1, 1, 1], cmi.PREC_FP64) as ctx:
+        ctx.set_fields(n, xn)
+        pg, exg, crg = ctx.interact_batch(0, in_dir, p)
+        J, _ = ctx.get_fields()
+    assert (exg == exo).all()
+    assert (crg == cro).all()
+    assert _bitwise_equal(pg, po)
+    _close_J(J, Jo)
+
+
+@pytest.mark.parametrize("tile", [None, (8, 8, 8), (4, 8, 16)])
+def test_transport_packets_matches_golden(golden, tile):
+    g = golden("transport_32.npz")
+    with cmi.Context(g["box"], g["ncell"], g["nsub"], cmi.PREC_FP64) as ctx:
+        if tile is not None:
+            ctx.set_engine(cmi.ENGINE_WALK, tile=tile)
+        for k in range(2):
+            print("transport golden case", k, "tile", tile, flush=True)
+            ctx.set_fields(g["n_%d" % k], g["xn_%d" % k])
+            ctx.reset_intensity()
+            p, st, cr, hp, stats = ctx.transport_packets(
+                g["photons_in_%d" % k], g["start_sub_%d" % k])
+            assert (st == g["status_%d" % k]).all()
+            assert (cr == g["crossings_%d" % k]).all()
+            assert (hp == g["hops_%d" % k]).all()
+            assert _bitwise_equal(p, g["photons_out_%d" % k])
+            assert stats["crossings"] == int(g["crossings_%d" % k].sum())
+            J, _ = ctx.get_fields()
+            _close_J(J, g["J_%d" % k])
+
+
+def _clumpy_grid(oracle, ncell, nsub, seed):
+    rng = np.random.default_rng(seed)
+    box = W.box_10pc()
+    og = O.OracleGrid(oracle, box, ncell, nsub)
+    og.n[:] = 1e8 * np.exp(rng.normal(0, 1, og.n.size))
+    og.xn[:] = 10 ** rng.uniform(-5, -1, og.n.size)
+    og.J[:] = 0
+    return box, og
+
+
+def test_emission_matches_oracle(oracle):
+    box, og = _clumpy_grid(oracle, [32, 32, 32], [4, 2, 1], 1)
+    pos, L = W.random_sources(5, box, seed=3)
+    with cmi.Context(box, [32, 32, 32], [4, 2, 1], cmi.PREC_FP64) as ctx:
+        ctx.set_sources(pos, L)
+        pg, sg = ctx.emit(100003, 17, 2)
+    po, so = og.emit(pos, L, 100003, 17, 2)
+    assert (sg == so).all()
+    assert (pg["pos"] == po["pos"]).all()
+    # device sincos / log against the host libm: a few ulp
+    assert np.allclose(pg["dir"], po["dir"], rtol=0, atol=4e-16)
+    assert np.allclose(pg["tau"], po["tau"], rtol=4e-16, atol=0)
+
+
+def test_transport_emitted_matches_oracle(oracle):
+    """cmi_transport (emission + walk) deposits what the oracle deposits for
+    the same packets."""
+    ncell, nsub = [32, 32, 32], [2, 2, 2]
+    box, og = _clumpy_grid(oracle, ncell, nsub, 2)
+    pos, L = W.random_sources(3, box, seed=4)
+    N = 60000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as ctx:
+        ctx.set_fields(og.n, og.xn)
+        ctx.set_sources(pos, L)
+        pk, ss = ctx.emit(N, 5, 0)
+        stats = ctx.transport(N, 5, 0)
+        J, _ = ctx.get_fields()
+    st, cr, hp = og.transport(pk, ss, nthreads=4)
+    assert stats["crossings"] == int(cr.sum())
+    assert stats["hops"] == int(hp.sum())
+    assert stats["absorbed"] == int((st == 0).sum())
+    assert stats["escaped"] == int((st == 1).sum())
+    assert stats["errors"] == 0
+    _close_J(J, og.J)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_ionization_matches_oracle(oracle, golden, form):
+    g = golden("neutral_fraction_16.npz")
+    with cmi.Context(g["box"], g["ncell"], [1, 1, 1], cmi.PREC_FP64) as ctx:
+        ctx.set_fields(g["n"], np.full(g["n"].size, 0.5))
+        ctx.set_intensity(g["J"])
+        l1 = ctx.update_ionization(float(g["L"]), int(g["N"]), 4e-19, form=form)
+        J, x = ctx.get_fields()
+    if form == 0:
+        assert (x == g["xn"]).all()  # the reference's own output
+    og = O.OracleGrid(oracle, g["box"], g["ncell"], [1, 1, 1])
+    og.n[:] = g["n"]
+    og.J[:] = g["J"]
+    og.compute_neutral_fraction(float(g["L"]), int(g["N"]), 4e-19, form=form)
+    assert (x == og.xn).all()
+    assert (J == 0).all()
+    assert np.isclose(l1, np.abs(og.xn - 0.5).mean(), rtol=1e-12)
+
+
+def test_multi_subgrid_ionization_layout(oracle):
+    """Tile-major device layout round-trips the reference layout."""
+    rng = np.random.default_rng(9)
+    box = W.box_10pc()
+    ncell, nsub = [32, 16, 32], [2, 1, 2]
+    nt = 32 * 16 * 32
+    n = 10 ** rng.uniform(7, 9, nt)
+    J = 10 ** rng.uniform(25, 33, nt)
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as ctx:
+        ctx.set_engine(cmi.ENGINE_WALK, tile=(4, 8, 2))
+        ctx.set_fields(n, None)
+        ctx.set_intensity(J)
+        J2, _ = ctx.get_fields()
+        assert (J2 == J).all()
+        ctx.update_ionization(4.26e49, 10 ** 7, 2.7e-19, form=1)
+        _, x = ctx.get_fields()
+    og = O.OracleGrid(oracle, box, ncell, nsub)
+    og.n[:] = n
+    og.J[:] = J
+    og.compute_neutral_fraction(4.26e49, 10 ** 7, 2.7e-19, form=1)
+    assert (x == og.xn).all()
+
+
+def test_fp32_transport_error_vs_fp64_oracle(oracle):
+    """FP32 traversal: same packets, error of the per-cell path sums against
+    the all-FP64 oracle stays at the FP32 rounding level."""
+    ncell, nsub = [32, 32, 32], [2, 2, 2]
+    box, og = _clumpy_grid(oracle, ncell, nsub, 3)
+    og.xn[:] = 10 ** np.random.default_rng(4).uniform(-6, -3, og.n.size)
+    pos, L = W.random_sources(2, box, seed=5)
+    N = 40000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as c64:
+        c64.set_sources(pos, L)
+        pk, ss = c64.emit(N, 8, 0)
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+        ctx.set_fields(og.n, og.xn)
+        p, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+        J, _ = ctx.get_fields()
+    q = pk.copy()
+    sto, cro, hpo = og.transport(q, ss, nthreads=4)
+    agree = (st == sto).mean()
+    assert agree > 0.999
+    same = st == sto
+    assert np.abs(cr[same].astype(int) - cro[same].astype(int)).max() <= 2
+    assert abs(J.sum() / og.J.sum() - 1) < 1e-5
+    big = og.J > 1e-3 * og.J.max()
+    rel = np.abs(J[big] - og.J[big]) / og.J[big]
+    assert np.median(rel) < 1e-5 and rel.max() < 1e-2
+
+
+@pytest.mark.parametrize("prec", [cmi.PREC_FP64, cmi.PREC_FP32])
+def test_stromgren_radius(prec):
+    """Uniform Stromgren sphere (64^3 cells as 4^3 subgrids of 16^3, n = 1e8,
+    Q = 4.26e49, alpha_B = 2.7e-19): the ionized volume-equivalent radius
+    agrees with Rs = (3Q / (4 pi n^2 alpha))^(1/3) within 2 %."""
+    box = W.box_10pc()
+    ncell, nsub = [64, 64, 64], [4, 4, 4]
+    N = 2_000_000
+    with cmi.Context(box, ncell, nsub, prec) as ctx:
+        ctx.set_sources([[0.0, 0.0, 0.0]], [W.Q_STROMGREN])
+        for it in range(16):
+            stats = ctx.transport(N, 1, it)
+            assert stats["errors"] == 0
+            l1 = ctx.update_ionization(W.Q_STROMGREN, N, W.ALPHA_B, form=1)
+            print("iteration", it, "L1", l1, "crossings", stats["crossings"],
+                  flush=True)
+        _, x = ctx.get_fields()
+    R = W.ionized_radius(x, box, ncell)
+    Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)
+    print("R/Rs", R / Rs)
+    assert abs(R / Rs - 1) < 0.02
