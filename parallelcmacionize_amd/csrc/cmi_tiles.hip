@@ -1,19 +1,946 @@
-// cmi_tiles.hip -- tiles engine (placeholder until the LDS-tile kernels land).
+// cmi_tiles.hip -- the tiles engine: packets binned per LDS tile, moved
+// between tiles in rounds.
+//
+// The reference moves packets between subgrids in 200-packet PhotonBuffers
+// tagged with the target subgrid and the entry TravelDirection
+// (PhotonBuffer.hpp:42-65, MemorySpace::add_photons, MemorySpace.hpp:92-125,
+// testDensitySubGrid_omp.cpp:1149-1329).  Here the unit of work is an LDS
+// tile (a power-of-two block of cells inside a subgrid) and one round moves
+// every packet in flight by one tile visit:
+//
+//   process   one workgroup per (tile, chunk of its packets): the tile's
+//             opacities are staged in LDS, every packet is walked until it
+//             leaves the tile or is absorbed, the path lengths accumulate in an
+//             LDS copy of the tile's J that is flushed to HBM once per visit;
+//             survivors are written to the workgroup's own output region
+//             (LDS-counter compaction) with their destination tile, and the
+//             per-face exit counts go to the next round's tile histogram;
+//   plan      one workgroup: exclusive scan of the histogram into tile
+//             segments, and the next round's work descriptors (tile, start,
+//             count) in chunks;
+//   scatter   destination tile of every survivor -> its slot in the tile's
+//             segment (wave-aggregated atomics on a per-tile cursor): the
+//             index list the next round gathers packets through.
+//
+// Refills are batched per wave: finished lanes wait until a threshold of
+// lanes is free (or the wave runs dry) and then write out and reload
+// together, so the divergent refill path is paid once per batch rather than
+// once per packet.
+#include <float.h>
+
 #include "cmi_ctx.hpp"
 
 namespace cmi {
 
-int tiles_transport_packets(cmi_ctx *c, cmi_photon *, const uint32_t *,
-                            size_t, int32_t *, uint32_t *, uint32_t *,
-                            cmi_stats *) {
-  return fail(c, CMI_ERR_STATE, "tiles engine not available");
+constexpr uint32_t kNone = 0xffffffffu;
+
+// Tile lattice of one grid: the global tile (tx, ty, tz) of a cell is
+// (gx >> tshift, ...); its device index is subgrid-major then tile-in-subgrid
+// (the order cell_addr() uses), so tile * tile_cells is its first cell.
+struct TileGeo {
+  int N[3];         // cells per axis (global)
+  int T[3];         // tile size per axis
+  int tshift[3];
+  int nts[3];       // tiles per subgrid per axis
+  int nsub[3];
+  int ntl[3];       // tiles per axis (global)
+  int tiles_per_sub;
+  int tile_cells;
+  int ntiles;
+  int pad_;
+  double box[3];
+  double inv_cell_size[3];
+  double unit_scale[3];
+};
+
+__device__ __forceinline__ int tile_index(const TileGeo &G, int tx, int ty,
+                                          int tz) {
+  const int sx = tx / G.nts[0], sy = ty / G.nts[1], sz = tz / G.nts[2];
+  const int sub = (sx * G.nsub[1] + sy) * G.nsub[2] + sz;
+  const int t = ((tx - sx * G.nts[0]) * G.nts[1] + (ty - sy * G.nts[1])) *
+                    G.nts[2] +
+                (tz - sz * G.nts[2]);
+  return sub * G.tiles_per_sub + t;
 }
 
-int tiles_transport(cmi_ctx *c, uint64_t, uint64_t, uint32_t, double,
-                    cmi_stats *) {
-  return fail(c, CMI_ERR_STATE, "tiles engine not available");
+__device__ __forceinline__ void tile_coords(const TileGeo &G, int tile, int &tx,
+                                            int &ty, int &tz) {
+  const int sub = tile / G.tiles_per_sub;
+  const int t = tile - sub * G.tiles_per_sub;
+  const int sx = sub / (G.nsub[1] * G.nsub[2]);
+  const int sy = (sub / G.nsub[2]) % G.nsub[1];
+  const int sz = sub % G.nsub[2];
+  const int lx = t / (G.nts[1] * G.nts[2]);
+  const int ly = (t / G.nts[2]) % G.nts[1];
+  const int lz = t % G.nts[2];
+  tx = sx * G.nts[0] + lx;
+  ty = sy * G.nts[1] + ly;
+  tz = sz * G.nts[2] + lz;
 }
 
-void tiles_destroy(cmi_ctx *) {}
+// FP32 packet record, 32 bytes: position relative to the tile origin in cell
+// units, direction scaled to cells per Delta_0, remaining optical depth, and
+// the global cell (10 bits per axis).
+struct Rec32 {
+  float4 a;  // px, py, pz, dx
+  float4 b;  // dy, dz, tau, cell bits
+};
+
+__device__ __forceinline__ uint32_t pack_cell(int x, int y, int z) {
+  return (uint32_t)x | ((uint32_t)y << 10) | ((uint32_t)z << 20);
+}
+__device__ __forceinline__ void unpack_cell(uint32_t c, int &x, int &y,
+                                            int &z) {
+  x = (int)(c & 1023u);
+  y = (int)((c >> 10) & 1023u);
+  z = (int)((c >> 20) & 1023u);
+}
+
+struct Desc {
+  int tile;
+  uint32_t start;
+  uint32_t count;
+  uint32_t pad;
+};
+
+// Per-packet outputs of explicit-list transports (indexed by packet id).
+struct PacketOut {
+  int32_t *status;
+  uint32_t *crossings;
+  uint32_t *hops;
+  float4 *final_a;  // last record (position in the final tile frame ...)
+  float4 *final_b;
+  int32_t *final_tile;
+};
+
+// ---------------------------------------------------------------------------
+// Emission straight into FP32 records (round-0 input).
+// ---------------------------------------------------------------------------
+__global__ void tiles_emit_kernel(TileGeo G, const SourcesDev *__restrict__ S,
+                                  uint64_t npackets, uint32_t iteration,
+                                  uint32_t key0, uint32_t key1,
+                                  Rec32 *__restrict__ out,
+                                  uint32_t *__restrict__ dest,
+                                  uint32_t *__restrict__ hist) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = k < npackets;
+  uint32_t tile = kNone;
+  if (valid) {
+    double pos[3], dir[3], tau;
+    emit_packet(*S, k, iteration, key0, key1, pos, dir, &tau);
+    int g[3];
+    double pc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      pc[a] = (pos[a] - G.box[a]) * G.inv_cell_size[a];
+      int c = (int)pc[a];
+      g[a] = c < 0 ? 0 : (c >= G.N[a] ? G.N[a] - 1 : c);
+    }
+    const int tx = g[0] >> G.tshift[0], ty = g[1] >> G.tshift[1],
+              tz = g[2] >> G.tshift[2];
+    tile = (uint32_t)tile_index(G, tx, ty, tz);
+    Rec32 r;
+    r.a.x = (float)(pc[0] - (double)(tx << G.tshift[0]));
+    r.a.y = (float)(pc[1] - (double)(ty << G.tshift[1]));
+    r.a.z = (float)(pc[2] - (double)(tz << G.tshift[2]));
+    r.a.w = (float)(dir[0] * G.unit_scale[0]);
+    r.b.x = (float)(dir[1] * G.unit_scale[1]);
+    r.b.y = (float)(dir[2] * G.unit_scale[2]);
+    r.b.z = (float)tau;
+    r.b.w = __uint_as_float(pack_cell(g[0], g[1], g[2]));
+    out[k] = r;
+    dest[k] = tile;
+  }
+  // wave-aggregated histogram: packets of a wave share one or two tiles
+  uint64_t todo = __ballot(valid);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const uint32_t t0 = __shfl(tile, leader, 64);
+    const uint64_t same = __ballot(valid && tile == t0);
+    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&hist[t0], (uint32_t)__popcll(same));
+    todo &= ~same;
+  }
+}
+
+// Explicit FP64 packet list -> FP32 records (start cell found like the
+// reference: subgrid by truncation, then the cell within it).
+__global__ void tiles_load_list_kernel(TileGeo G, GridDev g,
+                                       const PhotonRec *__restrict__ p,
+                                       const uint32_t *__restrict__ start_sub,
+                                       size_t n, Rec32 *__restrict__ out,
+                                       uint32_t *__restrict__ dest,
+                                       uint32_t *__restrict__ pid,
+                                       uint32_t *__restrict__ hist) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PhotonRec ph = p[i];
+  const int sub = (int)start_sub[i];
+  const int sx = sub / (G.nsub[1] * G.nsub[2]);
+  const int sy = (sub / G.nsub[2]) % G.nsub[1];
+  const int sz = sub % G.nsub[2];
+  const int so[3] = {sx * g.sc[0], sy * g.sc[1], sz * g.sc[2]};
+  const double *an = g.anchors + 3 * (size_t)sub;
+  int gc[3];
+  double pc[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double rel = (ph.pos[a] - an[a]) * g.inv_cell_size[a];
+    int c = (int)rel;
+    c = c < 0 ? 0 : (c >= g.sc[a] ? g.sc[a] - 1 : c);
+    gc[a] = so[a] + c;
+    pc[a] = rel + (double)so[a];
+  }
+  const int tx = gc[0] >> G.tshift[0], ty = gc[1] >> G.tshift[1],
+            tz = gc[2] >> G.tshift[2];
+  const uint32_t tile = (uint32_t)tile_index(G, tx, ty, tz);
+  Rec32 r;
+  r.a.x = (float)(pc[0] - (double)(tx << G.tshift[0]));
+  r.a.y = (float)(pc[1] - (double)(ty << G.tshift[1]));
+  r.a.z = (float)(pc[2] - (double)(tz << G.tshift[2]));
+  r.a.w = (float)(ph.dir[0] * G.unit_scale[0]);
+  r.b.x = (float)(ph.dir[1] * G.unit_scale[1]);
+  r.b.y = (float)(ph.dir[2] * G.unit_scale[2]);
+  r.b.z = (float)ph.tau;
+  r.b.w = __uint_as_float(pack_cell(gc[0], gc[1], gc[2]));
+  out[i] = r;
+  dest[i] = tile;
+  pid[i] = (uint32_t)i;
+  atomicAdd(&hist[tile], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// Plan: histogram -> tile segments and work descriptors (one workgroup).
+// ---------------------------------------------------------------------------
+constexpr int kPlanThreads = 1024;
+
+__device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v,
+                                                         uint64_t *sh,
+                                                         uint64_t *total) {
+  // inclusive scan within the wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t acc = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      const uint64_t t = sh[w];
+      sh[w] = acc;
+      acc += t;
+    }
+    sh[16] = acc;
+  }
+  __syncthreads();
+  const uint64_t r = sh[wave] + x - v;
+  *total = sh[16];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
+    int ntiles, uint32_t *__restrict__ hist, uint32_t *__restrict__ seg,
+    uint32_t *__restrict__ cursor, Desc *__restrict__ desc,
+    uint32_t *__restrict__ ndesc, uint32_t *__restrict__ total,
+    unsigned *__restrict__ ticket, uint32_t chunk_min, uint32_t max_chunks) {
+  __shared__ uint64_t sh[17];
+  const int per = (ntiles + kPlanThreads - 1) / kPlanThreads;
+  const int t0 = threadIdx.x * per;
+  const int t1 = min(ntiles, t0 + per);
+  uint64_t cnt = 0, nch = 0;
+  for (int t = t0; t < t1; ++t) {
+    const uint32_t c = hist[t];
+    cnt += c;
+    if (c) {
+      uint32_t chunk = (c + max_chunks - 1) / max_chunks;
+      chunk = chunk < chunk_min ? chunk_min : chunk;
+      nch += (c + chunk - 1) / chunk;
+    }
+  }
+  uint64_t tot_c, tot_d;
+  uint64_t base_c = block_exclusive_scan(cnt, sh, &tot_c);
+  uint64_t base_d = block_exclusive_scan(nch, sh, &tot_d);
+  for (int t = t0; t < t1; ++t) {
+    const uint32_t c = hist[t];
+    seg[t] = (uint32_t)base_c;
+    cursor[t] = 0;
+    hist[t] = 0;
+    if (c) {
+      uint32_t chunk = (c + max_chunks - 1) / max_chunks;
+      chunk = chunk < chunk_min ? chunk_min : chunk;
+      for (uint32_t s = 0; s < c; s += chunk) {
+        Desc d;
+        d.tile = t;
+        d.start = (uint32_t)base_c + s;
+        d.count = (c - s < chunk) ? (c - s) : chunk;
+        d.pad = 0;
+        desc[base_d++] = d;
+      }
+    }
+    base_c += c;
+  }
+  if (threadIdx.x == 0) {
+    *ndesc = (uint32_t)tot_d;
+    *total = (uint32_t)tot_c;
+    *ticket = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scatter: survivors' destination tiles -> index list sorted by tile.
+// ---------------------------------------------------------------------------
+__global__ void tiles_scatter_kernel(const uint32_t *__restrict__ nslots_ptr,
+                                     const uint32_t *__restrict__ dest,
+                                     const uint32_t *__restrict__ seg,
+                                     uint32_t *__restrict__ cursor,
+                                     uint32_t *__restrict__ idx) {
+  const uint32_t nslots = *nslots_ptr;
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nslots;
+       base += gridDim.x * blockDim.x) {
+    const uint32_t s = base + threadIdx.x;
+    const uint32_t t = (s < nslots) ? dest[s] : kNone;
+    uint64_t todo = __ballot(t != kNone);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const uint32_t t0 = __shfl(t, leader, 64);
+      const uint64_t same = __ballot(t == t0);
+      uint32_t b = 0;
+      if (lane == leader) b = atomicAdd(&cursor[t0], (uint32_t)__popcll(same));
+      b = __shfl(b, leader, 64);
+      if (t == t0) idx[seg[t0] + b + (uint32_t)__popcll(same & lt)] = s;
+      todo &= ~same;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Process: one tile visit per descriptor, FP32 DDA inside the tile.
+// ---------------------------------------------------------------------------
+template <class Acc>
+struct AccAtomic;
+template <>
+struct AccAtomic<float> {
+  __device__ static void add(float *p, float v) { atomicAdd(p, v); }
+};
+template <>
+struct AccAtomic<double> {
+  __device__ static void add(double *p, float v) { atomicAdd(p, (double)v); }
+};
+
+template <class Acc, int kThreads>
+__global__ void __launch_bounds__(kThreads) tiles_process_kernel(
+    TileGeo G, const float *__restrict__ kappa, double *__restrict__ J,
+    const Desc *__restrict__ desc, const uint32_t *__restrict__ ndesc_ptr,
+    unsigned *__restrict__ ticket, const Rec32 *__restrict__ Pin,
+    const uint32_t *__restrict__ idx, Rec32 *__restrict__ Pout,
+    uint32_t *__restrict__ dest_out, uint32_t *__restrict__ hist_next,
+    const uint32_t *__restrict__ pid_in, uint32_t *__restrict__ pid_out,
+    PacketOut po, Counters *__restrict__ cnt, int refill_threshold) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *kapL = reinterpret_cast<float *>(smem);
+  Acc *accL = reinterpret_cast<Acc *>(smem + sizeof(float) * G.tile_cells);
+  __shared__ uint32_t s_next, s_surv, s_desc;
+  __shared__ uint32_t s_face[6];
+
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t ndesc = *ndesc_ptr;
+  const int T0 = G.T[0], T1 = G.T[1], T2 = G.T[2];
+  const int st0 = T1 * T2, st1 = T2;
+  const int max_steps = T0 + T1 + T2 + 4;
+  unsigned long long my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0,
+                     my_err = 0;
+
+  for (;;) {
+    if (threadIdx.x == 0) s_desc = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t di = s_desc;
+    if (di >= ndesc) break;
+    const Desc D = desc[di];
+    const int tile = D.tile;
+    // stage the tile
+    {
+      const float4 *src =
+          reinterpret_cast<const float4 *>(kappa + (size_t)tile * G.tile_cells);
+      float4 *dst = reinterpret_cast<float4 *>(kapL);
+      for (int i = threadIdx.x; i < (G.tile_cells >> 2); i += kThreads)
+        dst[i] = src[i];
+      for (int i = threadIdx.x; i < G.tile_cells; i += kThreads) accL[i] = (Acc)0;
+      if (threadIdx.x == 0) {
+        s_next = 0;
+        s_surv = 0;
+      }
+      if (threadIdx.x < 6) s_face[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    int ttx, tty, ttz;
+    tile_coords(G, tile, ttx, tty, ttz);
+    const int ox = ttx << G.tshift[0], oy = tty << G.tshift[1],
+              oz = ttz << G.tshift[2];
+
+    // per-lane packet state
+    bool live = false, pending = false, exhausted = false;
+    int outcome = 0;  // 0 absorbed, 1 moved to a neighbour tile, 2 escaped, 3 error
+    int l0 = 0, l1 = 0, l2 = 0, off = 0, steps = 0, exit_axis = 0;
+    float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
+    float tm0 = 0.f, tm1 = 0.f, tm2 = 0.f, td0 = 0.f, td1 = 0.f, td2 = 0.f;
+    float t = 0.f, tau = 0.f;
+    uint32_t my_pid = 0, vis_cross = 0;
+
+    for (;;) {
+      const uint64_t fin = __ballot(pending);
+      const uint64_t idle = __ballot(!live && !pending && !exhausted);
+      const uint64_t lv = __ballot(live);
+      if (__popcll(fin) + __popcll(idle) >= refill_threshold || lv == 0) {
+        // ---- write out finished packets ----
+        if (pending) {
+          my_cross += vis_cross;
+          if (outcome == 0) {
+            ++my_abs;
+          } else if (outcome == 2) {
+            ++my_esc;
+          } else if (outcome == 3) {
+            ++my_err;
+          }
+          if (po.crossings) {
+            po.crossings[my_pid] += vis_cross;
+            po.hops[my_pid] += 1;
+            if (outcome != 1) po.status[my_pid] = outcome == 0 ? 0 : (outcome == 2 ? 1 : -2);
+          }
+        }
+        const bool moving = pending && outcome == 1;
+        const uint64_t mv = __ballot(moving);
+        uint32_t sbase = 0;
+        if (mv) {
+          const int leader = __ffsll((long long)mv) - 1;
+          if (lane == leader) sbase = atomicAdd(&s_surv, (uint32_t)__popcll(mv));
+          sbase = __shfl(sbase, leader, 64);
+        }
+        if (pending) {
+          // position at the exit, in the new tile's frame
+          const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
+          int gx = ox + l0, gy = oy + l1, gz = oz + l2;
+          if (moving) {
+            const int ntx = gx >> G.tshift[0], nty = gy >> G.tshift[1],
+                      ntz = gz >> G.tshift[2];
+            const float sx = (float)((ntx - ttx) * T0), sy = (float)((nty - tty) * T1),
+                        sz = (float)((ntz - ttz) * T2);
+            Rec32 r;
+            r.a.x = (exit_axis == 0) ? (d0 > 0.f ? 0.f : (float)T0) : q0 - sx;
+            r.a.y = (exit_axis == 1) ? (d1 > 0.f ? 0.f : (float)T1) : q1 - sy;
+            r.a.z = (exit_axis == 2) ? (d2 > 0.f ? 0.f : (float)T2) : q2 - sz;
+            r.a.w = d0;
+            r.b.x = d1;
+            r.b.y = d2;
+            r.b.z = tau;
+            r.b.w = __uint_as_float(pack_cell(gx, gy, gz));
+            const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
+            const int ntile = tile_index(G, ntx, nty, ntz);
+            Pout[slot] = r;
+            dest_out[slot] = (uint32_t)ntile;
+            if (pid_out) pid_out[slot] = my_pid;
+            const int face = 2 * exit_axis + ((exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2)) > 0.f ? 0 : 1);
+            atomicAdd(&s_face[face], 1u);
+          } else if (po.final_a) {
+            Rec32 r;
+            r.a = make_float4(q0, q1, q2, d0);
+            r.b = make_float4(d1, d2, tau, __uint_as_float(pack_cell(
+                                                 min(max(gx, 0), 1023), min(max(gy, 0), 1023),
+                                                 min(max(gz, 0), 1023))));
+            po.final_a[my_pid] = r.a;
+            po.final_b[my_pid] = r.b;
+            po.final_tile[my_pid] = tile;
+          }
+          pending = false;
+        }
+        // ---- reload ----
+        const bool need = !live && !exhausted;
+        const uint64_t nm = __ballot(need);
+        if (nm) {
+          const int leader = __ffsll((long long)nm) - 1;
+          uint32_t kb = 0;
+          if (lane == leader) kb = atomicAdd(&s_next, (uint32_t)__popcll(nm));
+          kb = __shfl(kb, leader, 64);
+          if (need) {
+            const uint32_t k = kb + (uint32_t)__popcll(nm & lt);
+            if (k < D.count) {
+              const uint32_t src = idx ? idx[D.start + k] : D.start + k;
+              const Rec32 r = Pin[src];
+              my_pid = pid_in ? pid_in[src] : 0u;
+              p0 = r.a.x;
+              p1 = r.a.y;
+              p2 = r.a.z;
+              d0 = r.a.w;
+              d1 = r.b.x;
+              d2 = r.b.y;
+              tau = r.b.z;
+              int gx, gy, gz;
+              unpack_cell(__float_as_uint(r.b.w), gx, gy, gz);
+              l0 = gx - ox;
+              l1 = gy - oy;
+              l2 = gz - oz;
+              ++my_hops;
+              vis_cross = 0;
+              steps = 0;
+              t = 0.f;
+              if ((unsigned)l0 >= (unsigned)T0 || (unsigned)l1 >= (unsigned)T1 ||
+                  (unsigned)l2 >= (unsigned)T2) {
+                pending = true;  // a record that does not belong here
+                outcome = 3;
+              } else {
+                const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
+                            i2 = __builtin_amdgcn_rcpf(d2);
+                tm0 = d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
+                               : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX);
+                tm1 = d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
+                               : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX);
+                tm2 = d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
+                               : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX);
+                td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
+                td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
+                td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+                off = (l0 * T1 + l1) * T2 + l2;
+                live = true;
+              }
+            } else {
+              exhausted = true;
+            }
+          }
+        }
+        if (__ballot(live) == 0 && __ballot(pending) == 0) break;
+      }
+      if (!live) continue;
+      // ---- one crossing ----
+      const float tn = fminf(tm0, fminf(tm1, tm2));
+      const float seg = fmaxf(tn - t, 0.f);
+      const float kap = kapL[off];
+      const float dtau = seg * kap;
+      ++vis_cross;
+      if (dtau >= tau) {
+        const float sa = kap > 0.f ? tau / kap : 0.f;
+        AccAtomic<Acc>::add(&accL[off], sa);
+        t = t + sa;
+        live = false;
+        pending = true;
+        outcome = 0;
+        continue;
+      }
+      tau -= dtau;
+      AccAtomic<Acc>::add(&accL[off], seg);
+      t = fmaxf(t, tn);
+      const bool a0 = tm0 == tn;
+      const bool a1 = !a0 && (tm1 == tn);
+      const bool a2 = !a0 && !a1;
+      const int s0 = d0 > 0.f ? 1 : -1, s1 = d1 > 0.f ? 1 : -1,
+                s2 = d2 > 0.f ? 1 : -1;
+      l0 += a0 ? s0 : 0;
+      l1 += a1 ? s1 : 0;
+      l2 += a2 ? s2 : 0;
+      off += a0 ? s0 * st0 : (a1 ? s1 * st1 : s2);
+      tm0 += a0 ? td0 : 0.f;
+      tm1 += a1 ? td1 : 0.f;
+      tm2 += a2 ? td2 : 0.f;
+      const bool out0 = (unsigned)l0 >= (unsigned)T0, out1 = (unsigned)l1 >= (unsigned)T1,
+                 out2 = (unsigned)l2 >= (unsigned)T2;
+      if (out0 | out1 | out2) {
+        exit_axis = out0 ? 0 : (out1 ? 1 : 2);
+        const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
+        const bool in_box = (unsigned)gx < (unsigned)G.N[0] &&
+                            (unsigned)gy < (unsigned)G.N[1] &&
+                            (unsigned)gz < (unsigned)G.N[2];
+        outcome = in_box ? 1 : 2;
+        live = false;
+        pending = true;
+      } else if (++steps > max_steps) {
+        outcome = 3;
+        live = false;
+        pending = true;
+      }
+    }
+    __syncthreads();
+    // flush the visit: J deposits, tail of the output region, face counts
+    {
+      double *Jt = J + (size_t)tile * G.tile_cells;
+      for (int i = threadIdx.x; i < G.tile_cells; i += kThreads) {
+        const Acc v = accL[i];
+        if (v != (Acc)0) atomicAdd(&Jt[i], (double)v);
+      }
+      const uint32_t ns = s_surv;
+      for (uint32_t i = ns + threadIdx.x; i < D.count; i += kThreads)
+        dest_out[D.start + i] = kNone;
+      if (threadIdx.x < 6) {
+        const uint32_t c = s_face[threadIdx.x];
+        if (c) {
+          const int ax = threadIdx.x >> 1;
+          const int sg = (threadIdx.x & 1) ? -1 : 1;
+          const int nt = tile_index(G, ttx + (ax == 0 ? sg : 0),
+                                    tty + (ax == 1 ? sg : 0),
+                                    ttz + (ax == 2 ? sg : 0));
+          atomicAdd(&hist_next[nt], c);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // kernel-wide counters
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    my_cross += __shfl_xor(my_cross, o, 64);
+    my_hops += __shfl_xor(my_hops, o, 64);
+    my_abs += __shfl_xor(my_abs, o, 64);
+    my_esc += __shfl_xor(my_esc, o, 64);
+    my_err += __shfl_xor(my_err, o, 64);
+  }
+  if (lane == 0) {
+    if (my_cross) atomicAdd(&cnt->crossings, my_cross);
+    if (my_hops) atomicAdd(&cnt->hops, my_hops);
+    if (my_abs) atomicAdd(&cnt->absorbed, my_abs);
+    if (my_esc) atomicAdd(&cnt->escaped, my_esc);
+    if (my_err) atomicAdd(&cnt->errors, my_err);
+  }
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+__global__ void zero_counters_kernel(Counters *c) {
+  if (threadIdx.x < sizeof(Counters) / 8)
+    reinterpret_cast<unsigned long long *>(c)[threadIdx.x] = 0;
+}
+
+// Final FP32 records of an explicit-list transport -> absolute positions.
+__global__ void tiles_finish_list_kernel(TileGeo G, GridDev g, size_t n,
+                                         const float4 *__restrict__ fa,
+                                         const int32_t *__restrict__ ftile,
+                                         const float4 *__restrict__ fb,
+                                         PhotonRec *__restrict__ p) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int tile = ftile[i];
+  if (tile < 0) return;
+  int tx, ty, tz;
+  tile_coords(G, tile, tx, ty, tz);
+  const float4 a = fa[i];
+  const float4 b = fb[i];
+  const int o[3] = {tx << G.tshift[0], ty << G.tshift[1], tz << G.tshift[2]};
+  const float q[3] = {a.x, a.y, a.z};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    p[i].pos[k] = G.box[k] + ((double)o[k] + (double)q[k]) * g.cell_size[k];
+  p[i].tau = (double)b.z;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct TilesEngine {
+  size_t cap = 0;         // packet capacity
+  int ntiles = 0;
+  size_t desc_cap = 0;
+  Rec32 *P[2] = {nullptr, nullptr};
+  uint32_t *dest[2] = {nullptr, nullptr};
+  uint32_t *pid[2] = {nullptr, nullptr};
+  uint32_t *idx = nullptr;
+  uint32_t *hist = nullptr, *seg = nullptr, *cursor = nullptr;
+  Desc *desc = nullptr;
+  uint32_t *ndesc = nullptr;
+  uint32_t *total = nullptr;   // [2]: packets of the round being planned
+  unsigned *ticket = nullptr;
+  uint32_t *h_total = nullptr;  // pinned
+  int tile_key[3] = {0, 0, 0};
+};
+
+static void free_engine(TilesEngine *e) {
+  if (!e) return;
+  for (int i = 0; i < 2; ++i) {
+    hipFree(e->P[i]);
+    hipFree(e->dest[i]);
+    hipFree(e->pid[i]);
+  }
+  hipFree(e->idx);
+  hipFree(e->hist);
+  hipFree(e->seg);
+  hipFree(e->cursor);
+  hipFree(e->desc);
+  hipFree(e->ndesc);
+  hipFree(e->total);
+  hipFree(e->ticket);
+  if (e->h_total) hipHostFree(e->h_total);
+  delete e;
+}
+
+void tiles_destroy(cmi_ctx *c) {
+  free_engine(c->tiles);
+  c->tiles = nullptr;
+}
+
+static TileGeo make_geo(const cmi_ctx *c) {
+  TileGeo G{};
+  for (int a = 0; a < 3; ++a) {
+    G.N[a] = c->ncell[a];
+    G.T[a] = c->tile[a];
+    G.tshift[a] = c->g.tshift[a];
+    G.nts[a] = c->g.ntile[a];
+    G.nsub[a] = c->nsub[a];
+    G.ntl[a] = c->ncell[a] / c->tile[a];
+    G.box[a] = c->box[a];
+    G.inv_cell_size[a] = c->g.inv_cell_size[a];
+    G.unit_scale[a] = c->g.unit_scale[a];
+  }
+  G.tiles_per_sub = G.nts[0] * G.nts[1] * G.nts[2];
+  G.tile_cells = c->g.tile_cells;
+  G.ntiles = G.tiles_per_sub * (int)c->nsubgrid;
+  return G;
+}
+
+static const uint32_t kChunkMin = 2048;
+static const uint32_t kMaxChunks = 96;
+
+static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
+  TilesEngine *e = c->tiles;
+  const int ntiles = make_geo(c).ntiles;
+  if (e && (e->cap < cap || e->ntiles != ntiles || (with_pid && !e->pid[0]) ||
+            e->tile_key[0] != c->tile[0] || e->tile_key[1] != c->tile[1] ||
+            e->tile_key[2] != c->tile[2])) {
+    free_engine(e);
+    c->tiles = e = nullptr;
+  }
+  if (e) return CMI_OK;
+  e = new TilesEngine();
+  e->cap = cap;
+  e->ntiles = ntiles;
+  e->desc_cap = (size_t)ntiles + cap / kChunkMin + 16;
+  for (int a = 0; a < 3; ++a) e->tile_key[a] = c->tile[a];
+  hipError_t err = hipSuccess;
+  for (int i = 0; i < 2 && err == hipSuccess; ++i) {
+    err = hipMalloc(&e->P[i], cap * sizeof(Rec32));
+    if (err == hipSuccess) err = hipMalloc(&e->dest[i], cap * sizeof(uint32_t));
+    if (err == hipSuccess && with_pid) err = hipMalloc(&e->pid[i], cap * sizeof(uint32_t));
+  }
+  if (err == hipSuccess) err = hipMalloc(&e->idx, cap * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->hist, ntiles * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->seg, ntiles * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->cursor, ntiles * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->desc, e->desc_cap * sizeof(Desc));
+  if (err == hipSuccess) err = hipMalloc(&e->ndesc, sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->total, 2 * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->ticket, sizeof(unsigned));
+  if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 2 * sizeof(uint32_t));
+  if (err != hipSuccess) {
+    free_engine(e);
+    return fail(c, CMI_ERR_ALLOC, "tiles engine: device allocation failed");
+  }
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(64), dim3(256), 0, c->stream, e->hist,
+                     (size_t)ntiles);
+  c->tiles = e;
+  return CMI_OK;
+}
+
+// Rounds of process / plan / scatter until no packet is left.  Round 0's
+// input (records, destinations, histogram) is already in P[0], dest[0],
+// hist; n0 packets.
+static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
+                      PacketOut po, uint32_t *rounds_out) {
+  TilesEngine *e = c->tiles;
+  const int threads = c->tuning.block_threads > 0 ? c->tuning.block_threads : 512;
+  const int refill = c->tuning.packets_per_block > 0 ? c->tuning.packets_per_block : 16;
+  const size_t lds = (size_t)G.tile_cells * (sizeof(float) + sizeof(double));
+  const int wg_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024)));
+  const int grid = c->num_cus * wg_per_cu;
+  const unsigned scatter_blocks = (unsigned)std::min<size_t>(4096, (n0 + 255) / 256 + 1);
+  // plan round 0
+  hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
+                     G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
+                     e->total, e->ticket, kChunkMin, kMaxChunks);
+  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(256), 0,
+                     c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
+  const int max_rounds = 4 * (G.ntl[0] + G.ntl[1] + G.ntl[2]) + 16;
+  int r = 0;
+  for (; r < max_rounds; ++r) {
+    // round r reads P[0] (through idx) and writes its survivors to P[1];
+    // the buffers swap roles before the next round
+    if (threads == 256) {
+      hipLaunchKernelGGL((tiles_process_kernel<double, 256>), dim3(grid), dim3(256),
+                         lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc,
+                         e->ticket, e->P[0], e->idx, e->P[1], e->dest[1],
+                         e->hist, with_pid ? e->pid[0] : nullptr,
+                         with_pid ? e->pid[1] : nullptr, po, c->d_cnt, refill);
+    } else {
+      hipLaunchKernelGGL((tiles_process_kernel<double, 512>), dim3(grid), dim3(512),
+                         lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc,
+                         e->ticket, e->P[0], e->idx, e->P[1], e->dest[1],
+                         e->hist, with_pid ? e->pid[0] : nullptr,
+                         with_pid ? e->pid[1] : nullptr, po, c->d_cnt, refill);
+    }
+    CMI_TRY_HIP(c, hipGetLastError());
+    // total[0] = packets processed this round = slots written to P[1];
+    // keep it in total[1] for the scatter, then plan the next round
+    CMI_TRY_HIP(c, hipMemcpyAsync(e->total + 1, e->total, sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
+                       G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
+                       e->total, e->ticket, kChunkMin, kMaxChunks);
+    CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, c->stream));
+    CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
+    if (e->h_total[0] == 0) {
+      ++r;
+      break;
+    }
+    hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(256), 0,
+                       c->stream, e->total + 1, e->dest[1], e->seg, e->cursor,
+                       e->idx);
+    CMI_TRY_HIP(c, hipGetLastError());
+    std::swap(e->P[0], e->P[1]);
+    std::swap(e->dest[0], e->dest[1]);
+    if (with_pid) std::swap(e->pid[0], e->pid[1]);
+  }
+  if (rounds_out) *rounds_out = (uint32_t)r;
+  if (r >= max_rounds) return fail(c, CMI_ERR_DIRECTION, "tiles engine: round limit reached");
+  return CMI_OK;
+}
+
+int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
+                    uint32_t iteration, double, cmi_stats *stats) {
+  if (c->precision != CMI_PREC_FP32)
+    return fail(c, CMI_ERR_ARG, "the tiles engine runs FP32 contexts");
+  if (npackets == 0 || npackets > 0xfffffff0ull)
+    return fail(c, CMI_ERR_ARG, "tiles engine: 1 .. 2^32-16 packets per call");
+  for (int a = 0; a < 3; ++a)
+    if (c->ncell[a] > 1024) return fail(c, CMI_ERR_ARG, "tiles engine: at most 1024 cells per axis");
+  int rc = ensure_engine(c, (size_t)npackets, false);
+  if (rc != CMI_OK) return rc;
+  TilesEngine *e = c->tiles;
+  const TileGeo G = make_geo(c);
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt);
+  CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
+  const unsigned eb = (unsigned)((npackets + 255) / 256);
+  hipLaunchKernelGGL(tiles_emit_kernel, dim3(eb), dim3(256), 0, c->stream, G, c->d_src,
+                     npackets, iteration, (uint32_t)seed, (uint32_t)(seed >> 32),
+                     e->P[0], e->dest[0], e->hist);
+  CMI_TRY_HIP(c, hipGetLastError());
+  PacketOut po{};
+  uint32_t rounds = 0;
+  rc = run_rounds(c, G, (size_t)npackets, false, po, &rounds);
+  if (rc != CMI_OK) return rc;
+  CMI_TRY_HIP(c, hipEventRecord(c->ev1, c->stream));
+  CMI_TRY_HIP(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(Counters),
+                                hipMemcpyDeviceToHost, c->stream));
+  CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  CMI_TRY_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->kernel_time_sum += ms * 1e-3;
+  c->kernel_calls += 1;
+  if (stats) {
+    stats->packets = npackets;
+    stats->crossings = c->h_cnt->crossings;
+    stats->hops = c->h_cnt->hops;
+    stats->absorbed = c->h_cnt->absorbed;
+    stats->escaped = c->h_cnt->escaped;
+    stats->errors = c->h_cnt->errors;
+    stats->rounds = rounds;
+    stats->reserved = 0;
+    stats->seconds = ms * 1e-3;
+  }
+  if (c->h_cnt->errors)
+    return fail(c, CMI_ERR_DIRECTION, "tiles engine: packets stopped on a walk bound");
+  return CMI_OK;
+}
+
+int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub,
+                            size_t n, int32_t *status, uint32_t *crossings,
+                            uint32_t *hops, cmi_stats *stats) {
+  if (c->precision != CMI_PREC_FP32)
+    return fail(c, CMI_ERR_ARG, "the tiles engine runs FP32 contexts");
+  if (n > 0xfffffff0ull) return fail(c, CMI_ERR_ARG, "too many packets");
+  int rc = ensure_engine(c, n, true);
+  if (rc != CMI_OK) return rc;
+  TilesEngine *e = c->tiles;
+  const TileGeo G = make_geo(c);
+  PhotonRec *dp = nullptr;
+  uint32_t *ds = nullptr;
+  int32_t *dst = nullptr, *dtile = nullptr;
+  uint32_t *dc = nullptr, *dh = nullptr;
+  float4 *fa = nullptr, *fb = nullptr;
+  hipError_t err = hipMalloc(&dp, n * sizeof(PhotonRec));
+  if (err == hipSuccess) err = hipMalloc(&ds, n * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&dst, n * sizeof(int32_t));
+  if (err == hipSuccess) err = hipMalloc(&dtile, n * sizeof(int32_t));
+  if (err == hipSuccess) err = hipMalloc(&dc, n * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&dh, n * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&fa, n * sizeof(float4));
+  if (err == hipSuccess) err = hipMalloc(&fb, n * sizeof(float4));
+  auto cleanup = [&]() {
+    hipFree(dp); hipFree(ds); hipFree(dst); hipFree(dtile); hipFree(dc);
+    hipFree(dh); hipFree(fa); hipFree(fb);
+  };
+  if (err != hipSuccess) {
+    cleanup();
+    return fail(c, CMI_ERR_ALLOC, "tiles list transport: allocation failed");
+  }
+  err = hipMemcpyAsync(dp, p, n * sizeof(PhotonRec), hipMemcpyHostToDevice, c->stream);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(ds, start_sub, n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(dst, 0xff, n * sizeof(int32_t), c->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(dtile, 0xff, n * sizeof(int32_t), c->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(dc, 0, n * sizeof(uint32_t), c->stream);
+  if (err == hipSuccess) err = hipMemsetAsync(dh, 0, n * sizeof(uint32_t), c->stream);
+  if (err != hipSuccess) {
+    cleanup();
+    return fail(c, CMI_ERR_HIP, "tiles list transport: copy failed");
+  }
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt);
+  CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(tiles_load_list_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, c->stream, G, c->g, dp, ds, n, e->P[0], e->dest[0], e->pid[0],
+                     e->hist);
+  PacketOut po{dst, dc, dh, fa, fb, dtile};
+  uint32_t rounds = 0;
+  rc = run_rounds(c, G, n, true, po, &rounds);
+  if (rc != CMI_OK) {
+    cleanup();
+    return rc;
+  }
+  hipLaunchKernelGGL(tiles_finish_list_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, c->stream, G, c->g, n, fa, dtile, fb, dp);
+  CMI_TRY_HIP(c, hipEventRecord(c->ev1, c->stream));
+  err = hipMemcpyAsync(p, dp, n * sizeof(PhotonRec), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess && status)
+    err = hipMemcpyAsync(status, dst, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess && crossings)
+    err = hipMemcpyAsync(crossings, dc, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess && hops)
+    err = hipMemcpyAsync(hops, dh, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(Counters), hipMemcpyDeviceToHost, c->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+  float ms = 0.f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  cleanup();
+  if (err != hipSuccess) return fail(c, CMI_ERR_HIP, "tiles list transport failed");
+  if (stats) {
+    stats->packets = n;
+    stats->crossings = c->h_cnt->crossings;
+    stats->hops = c->h_cnt->hops;
+    stats->absorbed = c->h_cnt->absorbed;
+    stats->escaped = c->h_cnt->escaped;
+    stats->errors = c->h_cnt->errors;
+    stats->rounds = rounds;
+    stats->reserved = 0;
+    stats->seconds = ms * 1e-3;
+  }
+  if (c->h_cnt->errors)
+    return fail(c, CMI_ERR_DIRECTION, "tiles engine: packets stopped on a walk bound");
+  return CMI_OK;
+}
 
 }  // namespace cmi

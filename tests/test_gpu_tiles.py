@@ -1,0 +1,104 @@
+"""GPU checks of the tiles engine (FP32 DDA inside LDS tiles, packets binned
+per tile between rounds) against the all-FP64 oracle.  The FP32 path is an
+approximation of the reference arithmetic, so the bars are statistical: the
+fate of each packet (absorbed / escaped) agrees for >= 99.9 % of packets,
+per-packet crossing counts within 2 (ties at cell edges are walked one axis at
+a time), and the per-cell path sums within FP32 rounding (median relative
+error < 1e-5, total < 1e-5)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import parallelcmacionize_amd as cmi
+from parallelcmacionize_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _field(seed, ncell, nsub, xlo=-6, xhi=-3):
+    rng = np.random.default_rng(seed)
+    nt = int(np.prod(ncell))
+    n = 1e8 * np.exp(rng.normal(0, 1, nt))
+    xn = 10 ** rng.uniform(xlo, xhi, nt)
+    return n, xn
+
+
+@pytest.mark.parametrize("tile", [(16, 16, 16), (8, 8, 8), (4, 8, 16)])
+def test_tiles_list_matches_fp64_oracle(oracle, tile):
+    ncell, nsub = [32, 32, 32], [2, 2, 2]
+    box = W.box_10pc()
+    n, xn = _field(3, ncell, nsub)
+    pos, L = W.random_sources(3, box, seed=5)
+    N = 50000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as c64:
+        c64.set_sources(pos, L)
+        pk, ss = c64.emit(N, 8, 0)
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+        ctx.set_engine(cmi.ENGINE_TILES, tile=tile)
+        ctx.set_fields(n, xn)
+        p, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+        J, _ = ctx.get_fields()
+    og = O.OracleGrid(oracle, box, ncell, nsub)
+    og.n[:] = n
+    og.xn[:] = xn
+    og.J[:] = 0
+    q = pk.copy()
+    sto, cro, hpo = og.transport(q, ss, nthreads=4)
+    assert stats["errors"] == 0
+    assert stats["absorbed"] + stats["escaped"] == N
+    assert (st == sto).mean() > 0.999
+    same = st == sto
+    assert np.abs(cr[same].astype(int) - cro[same].astype(int)).max() <= 2
+    assert abs(J.sum() / og.J.sum() - 1) < 1e-5
+    big = og.J > 1e-3 * og.J.max()
+    rel = np.abs(J[big] - og.J[big]) / og.J[big]
+    assert np.median(rel) < 1e-5 and rel.max() < 1e-2
+    # final positions: the same point up to FP32 rounding (cells of 0.3 pc)
+    d = np.linalg.norm(p["pos"][same] - q["pos"][same], axis=1)
+    assert np.median(d) < 1e-4 * (box[3] / 32)
+
+
+def test_tiles_transport_matches_walk(oracle):
+    """Emission + transport through the tiles engine and through the walk
+    engine (both FP32) deposit the same intensity field."""
+    ncell, nsub = [64, 64, 64], [4, 4, 4]
+    box = W.box_10pc()
+    n, xn = _field(4, ncell, nsub, -5, -2)
+    pos, L = W.random_sources(8, box, seed=3)
+    N = 400000
+    out = {}
+    for eng in (cmi.ENGINE_WALK, cmi.ENGINE_TILES):
+        with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+            ctx.set_engine(eng)
+            ctx.set_fields(n, xn)
+            ctx.set_sources(pos, L)
+            s = ctx.transport(N, 3, 1)
+            J, _ = ctx.get_fields()
+            out[eng] = (s, J)
+    sw, Jw = out[cmi.ENGINE_WALK]
+    st, Jt = out[cmi.ENGINE_TILES]
+    assert st["errors"] == 0 and st["absorbed"] + st["escaped"] == N
+    assert abs(st["absorbed"] - sw["absorbed"]) <= max(5, 1e-4 * N)
+    assert abs(st["crossings"] / sw["crossings"] - 1) < 2e-3
+    assert abs(Jt.sum() / Jw.sum() - 1) < 1e-5
+    big = Jw > 1e-3 * Jw.max()
+    rel = np.abs(Jt[big] - Jw[big]) / Jw[big]
+    assert np.median(rel) < 1e-5
+
+
+def test_tiles_stromgren_radius():
+    box = W.box_10pc()
+    ncell, nsub = [64, 64, 64], [4, 4, 4]
+    N = 2_000_000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+        ctx.set_engine(cmi.ENGINE_TILES)
+        ctx.set_sources([[0.0, 0.0, 0.0]], [W.Q_STROMGREN])
+        for it in range(16):
+            s = ctx.transport(N, 1, it)
+            assert s["errors"] == 0
+            l1 = ctx.update_ionization(W.Q_STROMGREN, N, W.ALPHA_B, form=1)
+        _, x = ctx.get_fields()
+    R = W.ionized_radius(x, box, ncell)
+    Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)
+    print("tiles R/Rs", R / Rs, "rounds", s["rounds"])
+    assert abs(R / Rs - 1) < 0.02
