@@ -86,7 +86,7 @@ def cpu_baseline(box, ncell, nsub, n, xn, pos, L, seed, iteration,
     S0 = 20000
     pk, ss, cr0 = sample(S0)
     t_small = run(pk, ss)
-    S = int(min(4_000_000, max(S0, S0 * target_s / max(t_small, 1e-3))))
+    S = int(min(20_000_000, max(S0, S0 * target_s / max(t_small, 1e-3))))
     pk, ss, cr = sample(S)
     t = run(pk, ss)
     c64.close()

@@ -78,13 +78,18 @@ typedef struct {
   double seconds;     /* device time of the transport (HIP events) */
 } cmi_stats;
 
-/* Tuning of the tiles engine (0 = default for any field). */
+/* Tuning of the engines (0 = default for any field). */
 typedef struct {
-  int tile[3];           /* LDS tile in cells (powers of two dividing the
-                            subgrid) */
-  int block_threads;     /* threads per workgroup */
-  int packets_per_block; /* packets one workgroup visit processes */
-  int walk_blocks;       /* persistent grid of the walk engine */
+  int tile[3];       /* LDS tile in cells (powers of two dividing the
+                        subgrid) */
+  int block_threads; /* tiles: threads per workgroup (256 or 512) */
+  int refill_batch;  /* tiles: finished lanes a wave gathers before it writes
+                        them out and reloads together (1..64) */
+  int chunk_min;     /* tiles: smallest share of a tile's packets one
+                        workgroup visit takes */
+  int walk_blocks;   /* walk: persistent grid size */
+  int lds_fp32;      /* tiles: 1 = FP32 LDS accumulators for J (flushed into
+                        FP64), 0 = FP64 */
 } cmi_tuning;
 
 typedef struct cmi_ctx cmi_ctx;

@@ -55,7 +55,8 @@ STATS_FIELDS = [f for f, _ in Stats._fields_ if f != "reserved"]
 
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int * 3), ("block_threads", C.c_int),
-                ("packets_per_block", C.c_int), ("walk_blocks", C.c_int)]
+                ("refill_batch", C.c_int), ("chunk_min", C.c_int),
+                ("walk_blocks", C.c_int), ("lds_fp32", C.c_int)]
 
 
 _lib = None
@@ -174,15 +175,17 @@ class Context:
         if rc != 0:
             raise CMIError(rc, self.lib.cmi_last_error(self.h).decode())
 
-    def set_engine(self, engine, tile=None, block_threads=0,
-                   packets_per_block=0, walk_blocks=0):
+    def set_engine(self, engine, tile=None, block_threads=0, refill_batch=0,
+                   chunk_min=0, walk_blocks=0, lds_fp32=0):
         t = Tuning()
         if tile is not None:
             for a in range(3):
                 t.tile[a] = int(tile[a])
         t.block_threads = int(block_threads)
-        t.packets_per_block = int(packets_per_block)
+        t.refill_batch = int(refill_batch)
+        t.chunk_min = int(chunk_min)
         t.walk_blocks = int(walk_blocks)
+        t.lds_fp32 = int(lds_fp32)
         self._check(self.lib.cmi_set_engine(self.h, int(engine), C.byref(t)))
 
     def set_fields(self, nH=None, xn=None):

@@ -120,12 +120,9 @@ __global__ void tiles_emit_kernel(TileGeo G, const SourcesDev *__restrict__ S,
                                   uint64_t npackets, uint32_t iteration,
                                   uint32_t key0, uint32_t key1,
                                   Rec32 *__restrict__ out,
-                                  uint32_t *__restrict__ dest,
-                                  uint32_t *__restrict__ hist) {
+                                  uint32_t *__restrict__ dest) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = k < npackets;
-  uint32_t tile = kNone;
-  if (valid) {
+  if (k < npackets) {
     double pos[3], dir[3], tau;
     emit_packet(*S, k, iteration, key0, key1, pos, dir, &tau);
     int g[3];
@@ -138,7 +135,7 @@ __global__ void tiles_emit_kernel(TileGeo G, const SourcesDev *__restrict__ S,
     }
     const int tx = g[0] >> G.tshift[0], ty = g[1] >> G.tshift[1],
               tz = g[2] >> G.tshift[2];
-    tile = (uint32_t)tile_index(G, tx, ty, tz);
+    const uint32_t tile = (uint32_t)tile_index(G, tx, ty, tz);
     Rec32 r;
     r.a.x = (float)(pc[0] - (double)(tx << G.tshift[0]));
     r.a.y = (float)(pc[1] - (double)(ty << G.tshift[1]));
@@ -151,15 +148,27 @@ __global__ void tiles_emit_kernel(TileGeo G, const SourcesDev *__restrict__ S,
     out[k] = r;
     dest[k] = tile;
   }
-  // wave-aggregated histogram: packets of a wave share one or two tiles
-  uint64_t todo = __ballot(valid);
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const uint32_t t0 = __shfl(tile, leader, 64);
-    const uint64_t same = __ballot(valid && tile == t0);
-    if ((int)(threadIdx.x & 63) == leader) atomicAdd(&hist[t0], (uint32_t)__popcll(same));
-    todo &= ~same;
+}
+
+// Tile of every source's start cell (the cell the emission kernel files its
+// packets under) gets that source's packet count.
+__global__ void tiles_source_hist_kernel(TileGeo G,
+                                         const SourcesDev *__restrict__ S,
+                                         uint32_t *__restrict__ hist) {
+  const int s = threadIdx.x;
+  if (s >= S->nsrc) return;
+  const uint64_t c = S->cum[s] - (s ? S->cum[s - 1] : 0);
+  if (c == 0) return;
+  int g[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double pc = (S->pos[s][a] - G.box[a]) * G.inv_cell_size[a];
+    const int cc = (int)pc;
+    g[a] = cc < 0 ? 0 : (cc >= G.N[a] ? G.N[a] - 1 : cc);
   }
+  const int tile = tile_index(G, g[0] >> G.tshift[0], g[1] >> G.tshift[1],
+                              g[2] >> G.tshift[2]);
+  atomicAdd(&hist[tile], (uint32_t)c);
 }
 
 // Explicit FP64 packet list -> FP32 records (start cell found like the
@@ -293,29 +302,70 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
 // ---------------------------------------------------------------------------
 // Scatter: survivors' destination tiles -> index list sorted by tile.
 // ---------------------------------------------------------------------------
-__global__ void tiles_scatter_kernel(const uint32_t *__restrict__ nslots_ptr,
-                                     const uint32_t *__restrict__ dest,
-                                     const uint32_t *__restrict__ seg,
-                                     uint32_t *__restrict__ cursor,
-                                     uint32_t *__restrict__ idx) {
+constexpr int kScatterThreads = 256;
+constexpr int kScatterItems = 16;                 // slots per thread
+constexpr int kScatterHash = 512;                 // LDS hash entries
+
+// One block takes kScatterThreads * kScatterItems consecutive slots: the
+// slots' destination tiles are counted in an LDS hash table (the rank of
+// each slot within its (block, tile) group comes from the LDS atomic), one
+// global atomic per distinct tile reserves the block's range in that tile's
+// segment, and the slots are written to the index list.  Consecutive slots
+// come from a few workgroup regions, so a block sees few distinct tiles and
+// the per-tile cursors see one atomic per block instead of one per wave.
+__global__ void __launch_bounds__(kScatterThreads) tiles_scatter_kernel(
+    const uint32_t *__restrict__ nslots_ptr, const uint32_t *__restrict__ dest,
+    const uint32_t *__restrict__ seg, uint32_t *__restrict__ cursor,
+    uint32_t *__restrict__ idx) {
+  __shared__ uint32_t keys[kScatterHash];
+  __shared__ uint32_t cnts[kScatterHash];
+  __shared__ uint32_t bases[kScatterHash];
   const uint32_t nslots = *nslots_ptr;
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t base = blockIdx.x * blockDim.x; base < nslots;
-       base += gridDim.x * blockDim.x) {
-    const uint32_t s = base + threadIdx.x;
-    const uint32_t t = (s < nslots) ? dest[s] : kNone;
-    uint64_t todo = __ballot(t != kNone);
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const uint32_t t0 = __shfl(t, leader, 64);
-      const uint64_t same = __ballot(t == t0);
-      uint32_t b = 0;
-      if (lane == leader) b = atomicAdd(&cursor[t0], (uint32_t)__popcll(same));
-      b = __shfl(b, leader, 64);
-      if (t == t0) idx[seg[t0] + b + (uint32_t)__popcll(same & lt)] = s;
-      todo &= ~same;
+  const uint32_t per_block = kScatterThreads * kScatterItems;
+  for (uint32_t base = blockIdx.x * per_block; base < nslots;
+       base += gridDim.x * per_block) {
+    for (int i = threadIdx.x; i < kScatterHash; i += kScatterThreads) {
+      keys[i] = kNone;
+      cnts[i] = 0;
     }
+    __syncthreads();
+    uint32_t t[kScatterItems];
+    int h[kScatterItems];
+    uint32_t rk[kScatterItems];
+#pragma unroll
+    for (int k = 0; k < kScatterItems; ++k) {
+      const uint32_t s = base + k * kScatterThreads + threadIdx.x;
+      t[k] = (s < nslots) ? dest[s] : kNone;
+      h[k] = -1;
+      rk[k] = 0;
+      if (t[k] != kNone) {
+        uint32_t hh = (t[k] * 2654435761u) >> 23;  // 9 bits
+        for (int probe = 0; probe < 32; ++probe) {
+          const uint32_t old = atomicCAS(&keys[hh], kNone, t[k]);
+          if (old == kNone || old == t[k]) {
+            h[k] = (int)hh;
+            rk[k] = atomicAdd(&cnts[hh], 1u);
+            break;
+          }
+          hh = (hh + 1) & (kScatterHash - 1);
+        }
+        if (h[k] < 0) rk[k] = atomicAdd(&cursor[t[k]], 1u);  // table full
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kScatterHash; i += kScatterThreads) {
+      const uint32_t c = cnts[i];
+      if (c) bases[i] = atomicAdd(&cursor[keys[i]], c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScatterItems; ++k) {
+      if (t[k] == kNone) continue;
+      const uint32_t s = base + k * kScatterThreads + threadIdx.x;
+      const uint32_t b = (h[k] >= 0) ? bases[h[k]] + rk[k] : rk[k];
+      idx[seg[t[k]] + b] = s;
+    }
+    __syncthreads();
   }
 }
 
@@ -333,7 +383,7 @@ struct AccAtomic<double> {
   __device__ static void add(double *p, float v) { atomicAdd(p, (double)v); }
 };
 
-template <class Acc, int kThreads>
+template <class Acc, int kThreads, bool kList>
 __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     TileGeo G, const float *__restrict__ kappa, double *__restrict__ J,
     const Desc *__restrict__ desc, const uint32_t *__restrict__ ndesc_ptr,
@@ -341,7 +391,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     const uint32_t *__restrict__ idx, Rec32 *__restrict__ Pout,
     uint32_t *__restrict__ dest_out, uint32_t *__restrict__ hist_next,
     const uint32_t *__restrict__ pid_in, uint32_t *__restrict__ pid_out,
-    PacketOut po, Counters *__restrict__ cnt, int refill_threshold) {
+    PacketOut po, Counters *__restrict__ cnt, int refill_batch) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *kapL = reinterpret_cast<float *>(smem);
   Acc *accL = reinterpret_cast<Acc *>(smem + sizeof(float) * G.tile_cells);
@@ -373,21 +423,34 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         dst[i] = src[i];
       for (int i = threadIdx.x; i < G.tile_cells; i += kThreads) accL[i] = (Acc)0;
       if (threadIdx.x == 0) {
-        s_next = 0;
+        s_next = kThreads;  // the first kThreads packets are claimed below
         s_surv = 0;
       }
       if (threadIdx.x < 6) s_face[threadIdx.x] = 0;
     }
-    __syncthreads();
     int ttx, tty, ttz;
     tile_coords(G, tile, ttx, tty, ttz);
     const int ox = ttx << G.tshift[0], oy = tty << G.tshift[1],
               oz = ttz << G.tshift[2];
 
+    // prefetched next record of this lane
+    bool has_nxt = false;
+    float4 na = make_float4(0.f, 0.f, 0.f, 0.f), nb = na;
+    uint32_t npid = 0;
+    if (threadIdx.x < D.count) {
+      const uint32_t src = idx ? idx[D.start + threadIdx.x] : D.start + threadIdx.x;
+      na = Pin[src].a;
+      nb = Pin[src].b;
+      if (kList) npid = pid_in[src];
+      has_nxt = true;
+    }
+    __syncthreads();
+
     // per-lane packet state
-    bool live = false, pending = false, exhausted = false;
+    bool live = false, pending = false;
     int outcome = 0;  // 0 absorbed, 1 moved to a neighbour tile, 2 escaped, 3 error
     int l0 = 0, l1 = 0, l2 = 0, off = 0, steps = 0, exit_axis = 0;
+    int so0 = 0, so1 = 0, so2 = 0;  // signed LDS strides along the direction
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
     float tm0 = 0.f, tm1 = 0.f, tm2 = 0.f, td0 = 0.f, td1 = 0.f, td2 = 0.f;
     float t = 0.f, tau = 0.f;
@@ -395,20 +458,16 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
 
     for (;;) {
       const uint64_t fin = __ballot(pending);
-      const uint64_t idle = __ballot(!live && !pending && !exhausted);
+      const uint64_t ready = __ballot(!live && !pending && has_nxt);
       const uint64_t lv = __ballot(live);
-      if (__popcll(fin) + __popcll(idle) >= refill_threshold || lv == 0) {
+      if (__popcll(fin) + __popcll(ready) >= refill_batch || lv == 0) {
         // ---- write out finished packets ----
         if (pending) {
           my_cross += vis_cross;
-          if (outcome == 0) {
-            ++my_abs;
-          } else if (outcome == 2) {
-            ++my_esc;
-          } else if (outcome == 3) {
-            ++my_err;
-          }
-          if (po.crossings) {
+          my_abs += (outcome == 0);
+          my_esc += (outcome == 2);
+          my_err += (outcome == 3);
+          if (kList) {
             po.crossings[my_pid] += vis_cross;
             po.hops[my_pid] += 1;
             if (outcome != 1) po.status[my_pid] = outcome == 0 ? 0 : (outcome == 2 ? 1 : -2);
@@ -425,7 +484,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         if (pending) {
           // position at the exit, in the new tile's frame
           const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
-          int gx = ox + l0, gy = oy + l1, gz = oz + l2;
+          const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
           if (moving) {
             const int ntx = gx >> G.tshift[0], nty = gy >> G.tshift[1],
                       ntz = gz >> G.tshift[2];
@@ -441,79 +500,80 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             r.b.z = tau;
             r.b.w = __uint_as_float(pack_cell(gx, gy, gz));
             const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
-            const int ntile = tile_index(G, ntx, nty, ntz);
             Pout[slot] = r;
-            dest_out[slot] = (uint32_t)ntile;
-            if (pid_out) pid_out[slot] = my_pid;
-            const int face = 2 * exit_axis + ((exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2)) > 0.f ? 0 : 1);
-            atomicAdd(&s_face[face], 1u);
-          } else if (po.final_a) {
-            Rec32 r;
-            r.a = make_float4(q0, q1, q2, d0);
-            r.b = make_float4(d1, d2, tau, __uint_as_float(pack_cell(
-                                                 min(max(gx, 0), 1023), min(max(gy, 0), 1023),
-                                                 min(max(gz, 0), 1023))));
-            po.final_a[my_pid] = r.a;
-            po.final_b[my_pid] = r.b;
+            dest_out[slot] = (uint32_t)tile_index(G, ntx, nty, ntz);
+            if (kList) pid_out[slot] = my_pid;
+            const float de = exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2);
+            atomicAdd(&s_face[2 * exit_axis + (de > 0.f ? 0 : 1)], 1u);
+          } else if (kList) {
+            po.final_a[my_pid] = make_float4(q0, q1, q2, d0);
+            po.final_b[my_pid] = make_float4(d1, d2, tau, 0.f);
             po.final_tile[my_pid] = tile;
           }
           pending = false;
         }
-        // ---- reload ----
-        const bool need = !live && !exhausted;
-        const uint64_t nm = __ballot(need);
-        if (nm) {
-          const int leader = __ffsll((long long)nm) - 1;
+        // ---- take the prefetched record, prefetch the next one ----
+        const bool take = !live && has_nxt;
+        if (take) {
+          p0 = na.x;
+          p1 = na.y;
+          p2 = na.z;
+          d0 = na.w;
+          d1 = nb.x;
+          d2 = nb.y;
+          tau = nb.z;
+          my_pid = npid;
+          int gx, gy, gz;
+          unpack_cell(__float_as_uint(nb.w), gx, gy, gz);
+          l0 = gx - ox;
+          l1 = gy - oy;
+          l2 = gz - oz;
+          ++my_hops;
+          vis_cross = 0;
+          steps = 0;
+          t = 0.f;
+          has_nxt = false;
+          if ((unsigned)l0 >= (unsigned)T0 || (unsigned)l1 >= (unsigned)T1 ||
+              (unsigned)l2 >= (unsigned)T2) {
+            pending = true;  // a record that does not belong to this tile
+            outcome = 3;
+          } else {
+            const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
+                        i2 = __builtin_amdgcn_rcpf(d2);
+            tm0 = d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
+                           : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX);
+            tm1 = d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
+                           : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX);
+            tm2 = d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
+                           : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX);
+            td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
+            td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
+            td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+            so0 = d0 > 0.f ? st0 : -st0;
+            so1 = d1 > 0.f ? st1 : -st1;
+            so2 = d2 > 0.f ? 1 : -1;
+            off = (l0 * T1 + l1) * T2 + l2;
+            live = true;
+          }
+        }
+        const uint64_t tm = __ballot(take);
+        if (tm) {
+          const int leader = __ffsll((long long)tm) - 1;
           uint32_t kb = 0;
-          if (lane == leader) kb = atomicAdd(&s_next, (uint32_t)__popcll(nm));
+          if (lane == leader) kb = atomicAdd(&s_next, (uint32_t)__popcll(tm));
           kb = __shfl(kb, leader, 64);
-          if (need) {
-            const uint32_t k = kb + (uint32_t)__popcll(nm & lt);
+          if (take) {
+            const uint32_t k = kb + (uint32_t)__popcll(tm & lt);
             if (k < D.count) {
               const uint32_t src = idx ? idx[D.start + k] : D.start + k;
-              const Rec32 r = Pin[src];
-              my_pid = pid_in ? pid_in[src] : 0u;
-              p0 = r.a.x;
-              p1 = r.a.y;
-              p2 = r.a.z;
-              d0 = r.a.w;
-              d1 = r.b.x;
-              d2 = r.b.y;
-              tau = r.b.z;
-              int gx, gy, gz;
-              unpack_cell(__float_as_uint(r.b.w), gx, gy, gz);
-              l0 = gx - ox;
-              l1 = gy - oy;
-              l2 = gz - oz;
-              ++my_hops;
-              vis_cross = 0;
-              steps = 0;
-              t = 0.f;
-              if ((unsigned)l0 >= (unsigned)T0 || (unsigned)l1 >= (unsigned)T1 ||
-                  (unsigned)l2 >= (unsigned)T2) {
-                pending = true;  // a record that does not belong here
-                outcome = 3;
-              } else {
-                const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
-                            i2 = __builtin_amdgcn_rcpf(d2);
-                tm0 = d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
-                               : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX);
-                tm1 = d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
-                               : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX);
-                tm2 = d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
-                               : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX);
-                td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
-                td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
-                td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
-                off = (l0 * T1 + l1) * T2 + l2;
-                live = true;
-              }
-            } else {
-              exhausted = true;
+              na = Pin[src].a;
+              nb = Pin[src].b;
+              if (kList) npid = pid_in[src];
+              has_nxt = true;
             }
           }
         }
-        if (__ballot(live) == 0 && __ballot(pending) == 0) break;
+        if (__ballot(live || pending || has_nxt) == 0) break;
       }
       if (!live) continue;
       // ---- one crossing ----
@@ -523,7 +583,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const float dtau = seg * kap;
       ++vis_cross;
       if (dtau >= tau) {
-        const float sa = kap > 0.f ? tau / kap : 0.f;
+        const float sa = kap > 0.f ? __fdividef(tau, kap) : 0.f;
         AccAtomic<Acc>::add(&accL[off], sa);
         t = t + sa;
         live = false;
@@ -537,12 +597,10 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const bool a0 = tm0 == tn;
       const bool a1 = !a0 && (tm1 == tn);
       const bool a2 = !a0 && !a1;
-      const int s0 = d0 > 0.f ? 1 : -1, s1 = d1 > 0.f ? 1 : -1,
-                s2 = d2 > 0.f ? 1 : -1;
-      l0 += a0 ? s0 : 0;
-      l1 += a1 ? s1 : 0;
-      l2 += a2 ? s2 : 0;
-      off += a0 ? s0 * st0 : (a1 ? s1 * st1 : s2);
+      l0 += a0 ? (d0 > 0.f ? 1 : -1) : 0;
+      l1 += a1 ? (d1 > 0.f ? 1 : -1) : 0;
+      l2 += a2 ? (d2 > 0.f ? 1 : -1) : 0;
+      off += a0 ? so0 : (a1 ? so1 : so2);
       tm0 += a0 ? td0 : 0.f;
       tm1 += a1 ? td1 : 0.f;
       tm2 += a2 ? td2 : 0.f;
@@ -703,6 +761,7 @@ static TileGeo make_geo(const cmi_ctx *c) {
 }
 
 static const uint32_t kChunkMin = 2048;
+static const uint32_t kChunkFloor = 256;  // smallest chunk_min accepted
 static const uint32_t kMaxChunks = 96;
 
 static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
@@ -718,7 +777,7 @@ static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
   e = new TilesEngine();
   e->cap = cap;
   e->ntiles = ntiles;
-  e->desc_cap = (size_t)ntiles + cap / kChunkMin + 16;
+  e->desc_cap = (size_t)ntiles + cap / kChunkFloor + 16;
   for (int a = 0; a < 3; ++a) e->tile_key[a] = c->tile[a];
   hipError_t err = hipSuccess;
   for (int i = 0; i < 2 && err == hipSuccess; ++i) {
@@ -751,36 +810,51 @@ static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
 static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
                       PacketOut po, uint32_t *rounds_out) {
   TilesEngine *e = c->tiles;
-  const int threads = c->tuning.block_threads > 0 ? c->tuning.block_threads : 512;
-  const int refill = c->tuning.packets_per_block > 0 ? c->tuning.packets_per_block : 16;
-  const size_t lds = (size_t)G.tile_cells * (sizeof(float) + sizeof(double));
+  const int threads = c->tuning.block_threads == 256 ? 256 : 512;
+  const int refill = c->tuning.refill_batch > 0 ? std::min(64, c->tuning.refill_batch) : 16;
+  const uint32_t chunk_min = c->tuning.chunk_min > 0
+      ? std::max<uint32_t>(kChunkFloor, (uint32_t)c->tuning.chunk_min) : kChunkMin;
+  const bool fp32acc = c->tuning.lds_fp32 != 0;
+  const size_t lds = (size_t)G.tile_cells * (sizeof(float) + (fp32acc ? sizeof(float) : sizeof(double)));
   const int wg_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024)));
   const int grid = c->num_cus * wg_per_cu;
-  const unsigned scatter_blocks = (unsigned)std::min<size_t>(4096, (n0 + 255) / 256 + 1);
+  const size_t per_block = (size_t)kScatterThreads * kScatterItems;
+  const unsigned scatter_blocks =
+      (unsigned)std::min<size_t>(8192, (n0 + per_block - 1) / per_block + 1);
+  auto launch_process = [&]() {
+#define CMI_PROC(ACC, TH, LIST)                                                  \
+  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST>), dim3(grid), dim3(TH), \
+                     lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc,   \
+                     e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,   \
+                     LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po, \
+                     c->d_cnt, refill)
+    if (with_pid) {
+      if (fp32acc) {
+        if (threads == 256) CMI_PROC(float, 256, true); else CMI_PROC(float, 512, true);
+      } else {
+        if (threads == 256) CMI_PROC(double, 256, true); else CMI_PROC(double, 512, true);
+      }
+    } else {
+      if (fp32acc) {
+        if (threads == 256) CMI_PROC(float, 256, false); else CMI_PROC(float, 512, false);
+      } else {
+        if (threads == 256) CMI_PROC(double, 256, false); else CMI_PROC(double, 512, false);
+      }
+    }
+#undef CMI_PROC
+  };
   // plan round 0
   hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
                      G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
-                     e->total, e->ticket, kChunkMin, kMaxChunks);
-  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(256), 0,
-                     c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
+                     e->total, e->ticket, chunk_min, kMaxChunks);
+  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads),
+                     0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
   const int max_rounds = 4 * (G.ntl[0] + G.ntl[1] + G.ntl[2]) + 16;
   int r = 0;
   for (; r < max_rounds; ++r) {
     // round r reads P[0] (through idx) and writes its survivors to P[1];
     // the buffers swap roles before the next round
-    if (threads == 256) {
-      hipLaunchKernelGGL((tiles_process_kernel<double, 256>), dim3(grid), dim3(256),
-                         lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc,
-                         e->ticket, e->P[0], e->idx, e->P[1], e->dest[1],
-                         e->hist, with_pid ? e->pid[0] : nullptr,
-                         with_pid ? e->pid[1] : nullptr, po, c->d_cnt, refill);
-    } else {
-      hipLaunchKernelGGL((tiles_process_kernel<double, 512>), dim3(grid), dim3(512),
-                         lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc,
-                         e->ticket, e->P[0], e->idx, e->P[1], e->dest[1],
-                         e->hist, with_pid ? e->pid[0] : nullptr,
-                         with_pid ? e->pid[1] : nullptr, po, c->d_cnt, refill);
-    }
+    launch_process();
     CMI_TRY_HIP(c, hipGetLastError());
     // total[0] = packets processed this round = slots written to P[1];
     // keep it in total[1] for the scatter, then plan the next round
@@ -788,7 +862,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
                                   hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
                        G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
-                       e->total, e->ticket, kChunkMin, kMaxChunks);
+                       e->total, e->ticket, chunk_min, kMaxChunks);
     CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream));
     CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
@@ -796,8 +870,8 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
       ++r;
       break;
     }
-    hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(256), 0,
-                       c->stream, e->total + 1, e->dest[1], e->seg, e->cursor,
+    hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads),
+                       0, c->stream, e->total + 1, e->dest[1], e->seg, e->cursor,
                        e->idx);
     CMI_TRY_HIP(c, hipGetLastError());
     std::swap(e->P[0], e->P[1]);
@@ -826,7 +900,9 @@ int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   const unsigned eb = (unsigned)((npackets + 255) / 256);
   hipLaunchKernelGGL(tiles_emit_kernel, dim3(eb), dim3(256), 0, c->stream, G, c->d_src,
                      npackets, iteration, (uint32_t)seed, (uint32_t)(seed >> 32),
-                     e->P[0], e->dest[0], e->hist);
+                     e->P[0], e->dest[0]);
+  hipLaunchKernelGGL(tiles_source_hist_kernel, dim3(1), dim3(kMaxSources), 0,
+                     c->stream, G, c->d_src, e->hist);
   CMI_TRY_HIP(c, hipGetLastError());
   PacketOut po{};
   uint32_t rounds = 0;
