@@ -97,19 +97,45 @@ def cpu_baseline(box, ncell, nsub, n, xn, pos, L, seed, iteration,
                        "on %d threads" % (S, iteration, cr, t, nthreads))}
 
 
+def copy_calibration(nbytes):
+    """A plain device-to-device copy of known size: rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE on it gives the counter-to-bytes factor used to
+    turn the transport kernels' counters into HBM bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a = C.c_void_p()
+    b = C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or \
+            hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        sys.exit("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(2):
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # device to device
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": 2}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--engine", default="walk", choices=["walk", "tiles"])
+    ap.add_argument("--engine", default="tiles", choices=["walk", "tiles"])
     ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--ncell", type=int, default=256)
     ap.add_argument("--nsub", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0,
+                    help="only copy this many bytes device-to-device twice "
+                         "(reference kernel for the HBM traffic counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(int(args.copy_calibration))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU "
               "path (see DESIGN.md for how the transport would shard)",
@@ -190,7 +216,8 @@ def main():
             "frac": achieved / PEAK_HBM,
             "traffic": None,
             "kernel": "walk_emit_kernel" if args.engine == "walk"
-                      else "tiles kernels",
+                      else "tiles transport (emit + rounds of process/plan/"
+                           "scatter) per step",
             "bytes_model": "%d B/crossing + 64 B/subgrid visit" % b_cross,
             "avg_kernel_s": avg_kernel,
         },

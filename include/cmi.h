@@ -85,8 +85,8 @@ typedef struct {
   int block_threads; /* tiles: threads per workgroup (256 or 512) */
   int refill_batch;  /* tiles: finished lanes a wave gathers before it writes
                         them out and reloads together (1..64) */
-  int chunk_min;     /* tiles: smallest share of a tile's packets one
-                        workgroup visit takes */
+  int chunk_min;     /* tiles: packets of one tile one workgroup visit
+                        takes (a tile's packets are split in such chunks) */
   int walk_blocks;   /* walk: persistent grid size */
   int lds_fp32;      /* tiles: 1 = FP32 LDS accumulators for J (flushed into
                         FP64), 0 = FP64 */

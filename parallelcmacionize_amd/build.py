@@ -55,7 +55,26 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    build_host_programs()
     return LIB
+
+
+ROOT = os.path.dirname(PKG)
+HOST_PROGRAMS = {
+    # C++ programs that use the host interface (host/cmi.hpp) over libcmi.so
+    os.path.join(ROOT, "tests", "cpp", "test_facade"):
+        os.path.join(ROOT, "tests", "cpp", "test_facade.cpp"),
+}
+
+
+def build_host_programs():
+    for out, src in HOST_PROGRAMS.items():
+        rel = os.path.relpath(PKG, os.path.dirname(out))
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", out, src,
+               "-L" + PKG, "-lcmi", "-Wl,-rpath,$ORIGIN/" + rel]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
 
 
 if __name__ == "__main__":

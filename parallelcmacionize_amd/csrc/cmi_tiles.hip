@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     int ntiles, uint32_t *__restrict__ hist, uint32_t *__restrict__ seg,
     uint32_t *__restrict__ cursor, Desc *__restrict__ desc,
     uint32_t *__restrict__ ndesc, uint32_t *__restrict__ total,
-    unsigned *__restrict__ ticket, uint32_t chunk_min, uint32_t max_chunks) {
+    unsigned *__restrict__ ticket, uint32_t chunk) {
   __shared__ uint64_t sh[17];
   const int per = (ntiles + kPlanThreads - 1) / kPlanThreads;
   const int t0 = threadIdx.x * per;
@@ -264,11 +264,7 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
   for (int t = t0; t < t1; ++t) {
     const uint32_t c = hist[t];
     cnt += c;
-    if (c) {
-      uint32_t chunk = (c + max_chunks - 1) / max_chunks;
-      chunk = chunk < chunk_min ? chunk_min : chunk;
-      nch += (c + chunk - 1) / chunk;
-    }
+    nch += (c + chunk - 1) / chunk;
   }
   uint64_t tot_c, tot_d;
   uint64_t base_c = block_exclusive_scan(cnt, sh, &tot_c);
@@ -279,8 +275,6 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     cursor[t] = 0;
     hist[t] = 0;
     if (c) {
-      uint32_t chunk = (c + max_chunks - 1) / max_chunks;
-      chunk = chunk < chunk_min ? chunk_min : chunk;
       for (uint32_t s = 0; s < c; s += chunk) {
         Desc d;
         d.tile = t;
@@ -760,9 +754,8 @@ static TileGeo make_geo(const cmi_ctx *c) {
   return G;
 }
 
-static const uint32_t kChunkMin = 2048;
-static const uint32_t kChunkFloor = 256;  // smallest chunk_min accepted
-static const uint32_t kMaxChunks = 96;
+static const uint32_t kChunk = 8192;      // packets per workgroup visit
+static const uint32_t kChunkFloor = 256;  // smallest chunk accepted
 
 static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
   TilesEngine *e = c->tiles;
@@ -811,9 +804,9 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
                       PacketOut po, uint32_t *rounds_out) {
   TilesEngine *e = c->tiles;
   const int threads = c->tuning.block_threads == 256 ? 256 : 512;
-  const int refill = c->tuning.refill_batch > 0 ? std::min(64, c->tuning.refill_batch) : 16;
-  const uint32_t chunk_min = c->tuning.chunk_min > 0
-      ? std::max<uint32_t>(kChunkFloor, (uint32_t)c->tuning.chunk_min) : kChunkMin;
+  const int refill = c->tuning.refill_batch > 0 ? std::min(64, c->tuning.refill_batch) : 32;
+  const uint32_t chunk = c->tuning.chunk_min > 0
+      ? std::max<uint32_t>(kChunkFloor, (uint32_t)c->tuning.chunk_min) : kChunk;
   const bool fp32acc = c->tuning.lds_fp32 != 0;
   const size_t lds = (size_t)G.tile_cells * (sizeof(float) + (fp32acc ? sizeof(float) : sizeof(double)));
   const int wg_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024)));
@@ -846,7 +839,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
   // plan round 0
   hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
                      G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
-                     e->total, e->ticket, chunk_min, kMaxChunks);
+                     e->total, e->ticket, chunk);
   hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads),
                      0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
   const int max_rounds = 4 * (G.ntl[0] + G.ntl[1] + G.ntl[2]) + 16;
@@ -862,7 +855,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
                                   hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
                        G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
-                       e->total, e->ticket, chunk_min, kMaxChunks);
+                       e->total, e->ticket, chunk);
     CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream));
     CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));

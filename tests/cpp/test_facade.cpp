@@ -1,0 +1,199 @@
+// Exercises the C++ host interface (parallelcmacionize_amd/host/cmi.hpp) the
+// way the reference's own programs use DensitySubGrid: the 54 known answers
+// of testDensitySubGridDirections.cpp:105-167, per-packet versus per-buffer
+// interact, neighbour tables, and a short Stromgren run through shoot() and
+// compute_neutral_fraction().  Exit code 0 = pass.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../parallelcmacionize_amd/host/cmi.hpp"
+
+using namespace cmi;
+
+static int failures = 0;
+#define CHECK(cond, ...)                         \
+  do {                                           \
+    if (!(cond)) {                               \
+      std::fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__); \
+      std::fprintf(stderr, __VA_ARGS__);         \
+      std::fprintf(stderr, "\n");                \
+      ++failures;                                \
+    }                                            \
+  } while (0)
+
+static void check_direction(DensitySubGrid &g, int ix, int iy, int iz, int ref) {
+  const int ti[3] = {ix, iy, iz};
+  const int r = g.get_output_direction(ti);
+  CHECK(r == ref, "output direction of %d %d %d: %d != %d", ix, iy, iz, r, ref);
+}
+
+static void check_input(DensitySubGrid &g, int dir, double x, double y, double z,
+                        int rx, int ry, int rz) {
+  const double p[3] = {x, y, z};
+  int ti[3];
+  g.get_start_index(p, dir, ti);
+  CHECK(ti[0] == rx && ti[1] == ry && ti[2] == rz,
+        "start index of direction %d: %d %d %d", dir, ti[0], ti[1], ti[2]);
+}
+
+int main() {
+  try {
+    // ---- known answers on the unit box with 10^3 cells ----
+    const double ubox[6] = {0., 0., 0., 1., 1., 1.};
+    const int unc[3] = {10, 10, 10}, one[3] = {1, 1, 1};
+    DensitySubGridGrid unit(ubox, unc, one);
+    DensitySubGrid &tg = unit[0];
+    check_direction(tg, 4, 5, 3, TRAVELDIRECTION_INSIDE);
+    check_direction(tg, -1, 5, 3, TRAVELDIRECTION_FACE_X_N);
+    check_direction(tg, 10, 5, 3, TRAVELDIRECTION_FACE_X_P);
+    check_direction(tg, 4, -1, 3, TRAVELDIRECTION_FACE_Y_N);
+    check_direction(tg, 4, 10, 3, TRAVELDIRECTION_FACE_Y_P);
+    check_direction(tg, 4, 5, -1, TRAVELDIRECTION_FACE_Z_N);
+    check_direction(tg, 4, 5, 10, TRAVELDIRECTION_FACE_Z_P);
+    check_direction(tg, -1, -1, 3, TRAVELDIRECTION_EDGE_Z_NN);
+    check_direction(tg, -1, 10, 3, TRAVELDIRECTION_EDGE_Z_NP);
+    check_direction(tg, 10, -1, 3, TRAVELDIRECTION_EDGE_Z_PN);
+    check_direction(tg, 10, 10, 3, TRAVELDIRECTION_EDGE_Z_PP);
+    check_direction(tg, -1, 5, -1, TRAVELDIRECTION_EDGE_Y_NN);
+    check_direction(tg, -1, 5, 10, TRAVELDIRECTION_EDGE_Y_NP);
+    check_direction(tg, 10, 5, -1, TRAVELDIRECTION_EDGE_Y_PN);
+    check_direction(tg, 10, 5, 10, TRAVELDIRECTION_EDGE_Y_PP);
+    check_direction(tg, 4, -1, -1, TRAVELDIRECTION_EDGE_X_NN);
+    check_direction(tg, 4, -1, 10, TRAVELDIRECTION_EDGE_X_NP);
+    check_direction(tg, 4, 10, -1, TRAVELDIRECTION_EDGE_X_PN);
+    check_direction(tg, 4, 10, 10, TRAVELDIRECTION_EDGE_X_PP);
+    check_direction(tg, -1, -1, -1, TRAVELDIRECTION_CORNER_NNN);
+    check_direction(tg, -1, -1, 10, TRAVELDIRECTION_CORNER_NNP);
+    check_direction(tg, -1, 10, -1, TRAVELDIRECTION_CORNER_NPN);
+    check_direction(tg, -1, 10, 10, TRAVELDIRECTION_CORNER_NPP);
+    check_direction(tg, 10, -1, -1, TRAVELDIRECTION_CORNER_PNN);
+    check_direction(tg, 10, -1, 10, TRAVELDIRECTION_CORNER_PNP);
+    check_direction(tg, 10, 10, -1, TRAVELDIRECTION_CORNER_PPN);
+    check_direction(tg, 10, 10, 10, TRAVELDIRECTION_CORNER_PPP);
+    check_input(tg, TRAVELDIRECTION_INSIDE, 0.5, 0.5, 0.5, 5, 5, 5);
+    check_input(tg, TRAVELDIRECTION_FACE_X_N, 0., 0.5, 0.5, 0, 5, 5);
+    check_input(tg, TRAVELDIRECTION_FACE_X_P, 1., 0.5, 0.5, 9, 5, 5);
+    check_input(tg, TRAVELDIRECTION_FACE_Y_N, 0.5, 0., 0.5, 5, 0, 5);
+    check_input(tg, TRAVELDIRECTION_FACE_Y_P, 0.5, 1., 0.5, 5, 9, 5);
+    check_input(tg, TRAVELDIRECTION_FACE_Z_N, 0.5, 0.5, 0., 5, 5, 0);
+    check_input(tg, TRAVELDIRECTION_FACE_Z_P, 0.5, 0.5, 1., 5, 5, 9);
+    check_input(tg, TRAVELDIRECTION_EDGE_X_NN, 0.5, 0., 0., 5, 0, 0);
+    check_input(tg, TRAVELDIRECTION_EDGE_X_NP, 0.5, 0., 1., 5, 0, 9);
+    check_input(tg, TRAVELDIRECTION_EDGE_X_PN, 0.5, 1., 0., 5, 9, 0);
+    check_input(tg, TRAVELDIRECTION_EDGE_X_PP, 0.5, 1., 1., 5, 9, 9);
+    check_input(tg, TRAVELDIRECTION_EDGE_Y_NN, 0., 0.5, 0., 0, 5, 0);
+    check_input(tg, TRAVELDIRECTION_EDGE_Y_NP, 0., 0.5, 1., 0, 5, 9);
+    check_input(tg, TRAVELDIRECTION_EDGE_Y_PN, 1., 0.5, 0., 9, 5, 0);
+    check_input(tg, TRAVELDIRECTION_EDGE_Y_PP, 1., 0.5, 1., 9, 5, 9);
+    check_input(tg, TRAVELDIRECTION_EDGE_Z_NN, 0., 0., 0.5, 0, 0, 5);
+    check_input(tg, TRAVELDIRECTION_EDGE_Z_NP, 0., 1., 0.5, 0, 9, 5);
+    check_input(tg, TRAVELDIRECTION_EDGE_Z_PN, 1., 0., 0.5, 9, 0, 5);
+    check_input(tg, TRAVELDIRECTION_EDGE_Z_PP, 1., 1., 0.5, 9, 9, 5);
+    check_input(tg, TRAVELDIRECTION_CORNER_NNN, 0., 0., 0., 0, 0, 0);
+    check_input(tg, TRAVELDIRECTION_CORNER_NNP, 0., 0., 1., 0, 0, 9);
+    check_input(tg, TRAVELDIRECTION_CORNER_NPN, 0., 1., 0., 0, 9, 0);
+    check_input(tg, TRAVELDIRECTION_CORNER_NPP, 0., 1., 1., 0, 9, 9);
+    check_input(tg, TRAVELDIRECTION_CORNER_PNN, 1., 0., 0., 9, 0, 0);
+    check_input(tg, TRAVELDIRECTION_CORNER_PNP, 1., 0., 1., 9, 0, 9);
+    check_input(tg, TRAVELDIRECTION_CORNER_PPN, 1., 1., 0., 9, 9, 0);
+    check_input(tg, TRAVELDIRECTION_CORNER_PPP, 1., 1., 1., 9, 9, 9);
+
+    // ---- per-packet and per-buffer interact agree ----
+    const double pc = 3.086e16;
+    const double box[6] = {-5 * pc, -5 * pc, -5 * pc, 10 * pc, 10 * pc, 10 * pc};
+    const int ncell[3] = {32, 32, 32}, nsub[3] = {2, 2, 2};
+    DensitySubGridGrid grid(box, ncell, nsub);
+    for (size_t s = 0; s < grid.size(); ++s)
+      for (size_t i = 0; i < grid[s].get_number_of_cells(); ++i)
+        grid[s].set_neutral_fraction((unsigned)i, 1e-3 * (1 + (i % 7)));
+    PhotonBuffer buf;
+    buf.set_subgrid_index(7);
+    buf.set_direction(TRAVELDIRECTION_INSIDE);
+    std::vector<Photon> single;
+    for (unsigned int k = 0; k < PHOTONBUFFER_SIZE; ++k) {
+      Photon &p = buf[buf.get_next_free_photon()];
+      const double ct = 2. * ((k * 37 % 200) + 0.5) / 200. - 1.;
+      const double ph = 2. * M_PI * ((k * 53 % 200) + 0.5) / 200.;
+      const double st = std::sqrt(1. - ct * ct);
+      p.set_position(0.1 * pc, 0.2 * pc, 0.3 * pc);
+      p.set_direction(st * std::cos(ph), st * std::sin(ph), ct);
+      p.set_target_optical_depth(0.05 + 0.01 * (k % 10));
+      p.set_photoionization_cross_section(6.3e-22);
+      p.set_weight(1.);
+      single.push_back(p);
+    }
+    std::vector<int> exits(PHOTONBUFFER_SIZE);
+    grid[7].interact(buf, exits.data());
+    for (unsigned int k = 0; k < PHOTONBUFFER_SIZE; ++k) {
+      const int e = grid[7].interact(single[k], TRAVELDIRECTION_INSIDE);
+      CHECK(e == exits[k], "exit of packet %u: %d vs %d", k, e, exits[k]);
+      for (int a = 0; a < 3; ++a)
+        CHECK(single[k].get_position()[a] == buf[k].get_position()[a],
+              "position of packet %u", k);
+      CHECK(single[k].get_target_optical_depth() == buf[k].get_target_optical_depth(),
+            "tau of packet %u", k);
+    }
+    // neighbours (testDensitySubGrid_omp.cpp:1836-1874): subgrid 7 = (1,1,1)
+    CHECK(grid[7].get_neighbour(TRAVELDIRECTION_INSIDE) == 7, "self neighbour");
+    CHECK(grid[7].get_neighbour(TRAVELDIRECTION_CORNER_NNN) == 0, "corner neighbour");
+    CHECK(grid[7].get_neighbour(TRAVELDIRECTION_FACE_X_P) == NEIGHBOUR_OUTSIDE,
+          "outside neighbour");
+    CHECK(TravelDirections::output_to_input_direction(TRAVELDIRECTION_EDGE_Y_PN) ==
+              TRAVELDIRECTION_EDGE_Y_NP, "output_to_input");
+
+    // ---- MemorySpace spill ----
+    MemorySpace ms(8);
+    const size_t b0 = ms.get_free_buffer();
+    ms[b0].set_subgrid_index(3);
+    ms[b0].set_direction(TRAVELDIRECTION_FACE_Z_N);
+    PhotonBuffer half;
+    half.grow(150);
+    size_t b1 = ms.add_photons(b0, half);
+    CHECK(b1 == b0 && ms[b0].size() == 150, "first add");
+    b1 = ms.add_photons(b0, half);
+    CHECK(b1 != b0 && ms[b0].size() == 200 && ms[b1].size() == 100 &&
+              ms[b1].get_subgrid_index() == 3 &&
+              ms[b1].get_direction() == TRAVELDIRECTION_FACE_Z_N,
+          "spill");
+
+    // ---- Stromgren sphere through shoot() / compute_neutral_fraction() ----
+    const int sc3[3] = {64, 64, 64}, sn3[3] = {4, 4, 4};
+    DensitySubGridGrid sg(box, sc3, sn3, CMI_PREC_FP32);
+    sg.set_engine(CMI_ENGINE_TILES);
+    sg.set_sources({0., 0., 0.}, {4.26e49});
+    const uint64_t N = 2000000;
+    for (uint32_t it = 0; it < 16; ++it) {
+      sg.shoot(N, 1, it);
+      sg.compute_neutral_fraction(4.26e49, N, 2.7e-19, 1);
+    }
+    double ion = 0.;
+    for (size_t s = 0; s < sg.size(); ++s) {
+      const double *x = sg[s].get_neutral_fraction();
+      for (size_t i = 0; i < sg[s].get_number_of_cells(); ++i) ion += 1. - x[i];
+    }
+    const double dv = std::pow(10 * pc / 64, 3);
+    const double R = std::cbrt(3. * ion * dv / (4. * M_PI));
+    const double Rs = std::cbrt(3. * 4.26e49 / (4. * M_PI * 1e16 * 2.7e-19));
+    std::printf("Stromgren R/Rs = %.4f\n", R / Rs);
+    CHECK(std::fabs(R / Rs - 1.) < 0.02, "Stromgren radius %.4f", R / Rs);
+
+    // ---- errors come back as exceptions, not aborts ----
+    bool threw = false;
+    try {
+      grid[0].get_neighbour(99);
+    } catch (const Error &e) {
+      threw = e.code() == CMI_ERR_ARG;
+    }
+    CHECK(threw, "bad direction throws");
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "exception: %s\n", e.what());
+    return 2;
+  }
+  if (failures) {
+    std::fprintf(stderr, "%d failures\n", failures);
+    return 1;
+  }
+  std::printf("All tests successful.\n");
+  return 0;
+}
