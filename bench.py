@@ -216,12 +216,24 @@ def main():
             "frac": achieved / PEAK_HBM,
             "traffic": None,
             "kernel": "walk_emit_kernel" if args.engine == "walk"
-                      else "tiles transport (emit + rounds of process/plan/"
-                           "scatter) per step",
-            "bytes_model": "%d B/crossing + 64 B/subgrid visit" % b_cross,
+                      else "tiles_process_kernel",
+            "launches": calls,
+            "bytes_model": ("%d B/crossing + 64 B/%s visit, averaged per "
+                            "launch" % (b_cross, "subgrid" if args.engine ==
+                                        "walk" else "tile")),
             "avg_kernel_s": avg_kernel,
         },
     }
+    traffic_file = os.path.join(ROOT, "profiles",
+                                "tiles_process_kernel_256clumpy_traffic.json")
+    if (args.engine == "tiles" and ncell == 256 and N == 100_000_000 and
+            prec == cmi.PREC_FP32 and os.path.exists(traffic_file)):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        # HBM bytes per launch of the same kernel on the same workload, from
+        # the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = tr["source"]
     if not args.no_cpu:
         xn_now = ctx.get_fields()[1]
         nthreads = min(16, os.cpu_count() or 1)

@@ -200,9 +200,12 @@ int cmi_get_neighbours(const cmi_ctx *ctx, uint32_t subgrid,
  * DensitySubGrid.hpp:780, summed over subgrids). */
 int cmi_get_sizes(const cmi_ctx *ctx, uint64_t *nsubgrid, uint64_t *ncell);
 
-/* Device timing of the last kernel sequence on the context's stream, for
- * the roofline figure: average duration (s) of the dominant transport
- * kernel over the calls since the last reset, and the number of calls. */
+/* Device timing (HIP events on the context's stream) of the dominant
+ * transport kernel, for the roofline figure: its average launch duration (s)
+ * over the launches since the last reset, and the number of launches.  The
+ * dominant kernel is walk_emit_kernel (one launch per cmi_transport) for the
+ * walk engine and tiles_process_kernel (one launch per round) for the tiles
+ * engine. */
 int cmi_kernel_timing(cmi_ctx *ctx, double *avg_seconds, uint64_t *calls,
                       int reset);
 
