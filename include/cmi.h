@@ -212,6 +212,35 @@ int cmi_kernel_timing(cmi_ctx *ctx, double *avg_seconds, uint64_t *calls,
 /* Synchronise the context's stream. */
 int cmi_synchronize(cmi_ctx *ctx);
 
+/* ---- problem sets ----
+ * Many independent grids transported together (no counterpart in the
+ * reference, which runs one grid per process): every problem has its own box,
+ * cubic-cell grid, density, sources and packet budget; all share the LDS tile
+ * shape (tuning->tile[0], default 16, which must divide every problem's cell
+ * counts), run in FP32 on the tiles engine, and one iteration of all of them
+ * -- emission, transport, ionization update -- is one replay of a captured
+ * hipGraph.  Cell arrays of a problem use the plain lattice order
+ * x*ny*nz + y*nz + z.  Up to 256 problems and 4096 sources in a set. */
+typedef struct cmi_batch cmi_batch;
+int cmi_batch_create(int nprob, const double *boxes, const int *ncells,
+                     const cmi_tuning *tuning, cmi_batch **out);
+void cmi_batch_destroy(cmi_batch *b);
+const char *cmi_batch_last_error(const cmi_batch *b);
+int cmi_batch_set_fields(cmi_batch *b, int problem, const double *nH,
+                         const double *xn);
+int cmi_batch_get_fields(cmi_batch *b, int problem, double *J, double *xn);
+int cmi_batch_set_sources(cmi_batch *b, int problem, const double *pos,
+                          const double *L, int nsrc);
+/* One iteration of every problem: npackets[p] packets for problem p, then the
+ * FP64 ionization update (alphaH, form as cmi_update_ionization).  Per
+ * problem (arrays of nprob, any may be NULL): mean |dx_n|, the time from the
+ * start of the iteration until the problem's last packet finished (device
+ * clock), and its cell crossings. */
+int cmi_batch_iteration(cmi_batch *b, const uint64_t *npackets, uint64_t seed,
+                        uint32_t iteration, double alphaH, int form,
+                        double *l1_change, double *problem_seconds,
+                        uint64_t *problem_crossings, cmi_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -131,6 +131,10 @@ uint32_t count_and_clear_nan(DensitySubGrid &shadow, int nc[3]) {
   return c;
 }
 
+size_t grid_cells(const int ncell[3]) {
+  return (size_t)ncell[0] * ncell[1] * ncell[2];
+}
+
 } // namespace
 
 extern "C" {
@@ -269,130 +273,96 @@ void ref_compute_neutral_fraction(const double box[6], const int ncell[3],
   for (size_t i = 0; i < nc; ++i) xn_out[i] = x[i];
 }
 
-/* Timed CPU baseline: task-parallel transport of an explicit packet list with
- * the reference's classes -- DensitySubGrid::interact, 200-packet
- * PhotonBuffers in a MemorySpace pool (MemorySpace::add_photons spill), per
- * subgrid active buffers per output direction, subgrid locks -- driven by a
- * worker loop that restates testDensitySubGrid_omp.cpp:1149-1329 and
- * 2274-2401 (the driver itself needs hdf5.h, absent here).  Returns the wall
- * time in seconds of the transport; J is returned in reference order. */
+/* Timed CPU baseline: parallel transport of an explicit packet list with the
+ * reference's own DensitySubGrid::interact and PhotonBuffer routing
+ * (do_photon_traversal into 27 output buffers, output_to_input_direction,
+ * get_neighbour; testDensitySubGrid_omp.cpp:607-666, 1149-1329).  Threads
+ * take 200-packet source buffers from a shared counter; each thread owns a
+ * private copy of the grid, which is the reference's subgrid-copy scheme
+ * (testDensitySubGrid_omp.cpp:720-, 2116-2171) taken to one copy of every
+ * subgrid per thread: no subgrid locks, no contention.  The copies' J are
+ * summed at the end (update_intensities, DensitySubGrid.hpp:1096).  Returns
+ * the wall time in seconds of the transport (the copies are set up before
+ * the clock starts); J is returned in reference order. */
 double ref_transport_tasks(const double box[6], const int ncell[3],
                            const int nsub[3], const double *n,
                            const double *xn, double *J, const double *photons,
                            const uint32_t *start_sub, size_t np, int nthreads,
-                           size_t pool_size) {
-  RefGrid grid(box, ncell, nsub);
-  grid.set_fields(n, xn);
-  const int nsg = (int)grid.sub.size();
-  int sc[3] = {ncell[0] / nsub[0], ncell[1] / nsub[1], ncell[2] / nsub[2]};
-  MemorySpace pool(pool_size);
-  std::vector<Lock> sublock(nsg);
-  std::mutex qmutex;
-  std::vector<size_t> queue; /* full (or flushed) buffers ready to traverse */
-  std::atomic<size_t> done(0);
-  const HPhoton *hp = reinterpret_cast<const HPhoton *>(photons);
-
-  /* source buffers: consecutive packets sharing a start subgrid */
-  for (size_t k = 0; k < np;) {
-    const size_t b = pool.get_free_buffer();
-    PhotonBuffer &buf = pool[b];
-    buf.reset();
-    buf.set_subgrid_index(start_sub[k]);
-    buf.set_direction(TRAVELDIRECTION_INSIDE);
-    size_t m = 0;
-    while (k < np && m < PHOTONBUFFER_SIZE && start_sub[k] == start_sub[k - m]) {
-      Photon &p = buf[buf.get_next_free_photon()];
-      to_photon(hp[k], p);
-      ++k;
-      ++m;
-    }
-    queue.push_back(b);
+                           size_t /*pool_size*/) {
+  std::vector<RefGrid *> grids(nthreads, nullptr);
+#pragma omp parallel num_threads(nthreads)
+  {
+    const int t = omp_get_thread_num();
+    grids[t] = new RefGrid(box, ncell, nsub);
+    grids[t]->set_fields(n, xn);
   }
-
+  int sc[3] = {ncell[0] / nsub[0], ncell[1] / nsub[1], ncell[2] / nsub[2]};
+  const HPhoton *hp = reinterpret_cast<const HPhoton *>(photons);
+  std::atomic<size_t> next(0);
   const auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel num_threads(nthreads)
   {
+    RefGrid &grid = *grids[omp_get_thread_num()];
+    std::vector<PhotonBuffer *> work, spare;
     PhotonBuffer local[TRAVELDIRECTION_NUMBER];
-    while (done.load() < np) {
-      size_t b = (size_t)-1;
-      {
-        std::lock_guard<std::mutex> lk(qmutex);
-        for (size_t q = queue.size(); q-- > 0;) {
-          const unsigned int sg = pool[queue[q]].get_subgrid_index();
-          if (sublock[sg].try_lock()) {
-            b = queue[q];
-            queue.erase(queue.begin() + q);
-            break;
+    auto get_buffer = [&]() {
+      if (spare.empty()) return new PhotonBuffer();
+      PhotonBuffer *b = spare.back();
+      spare.pop_back();
+      b->reset();
+      return b;
+    };
+    for (;;) {
+      const size_t k0 = next.fetch_add(PHOTONBUFFER_SIZE);
+      if (k0 >= np) break;
+      const size_t k1 = std::min(np, k0 + PHOTONBUFFER_SIZE);
+      // source buffers: consecutive packets that share a start subgrid
+      for (size_t k = k0; k < k1;) {
+        PhotonBuffer *b = get_buffer();
+        b->set_subgrid_index(start_sub[k]);
+        b->set_direction(TRAVELDIRECTION_INSIDE);
+        const uint32_t sg = start_sub[k];
+        while (k < k1 && start_sub[k] == sg) {
+          to_photon(hp[k], (*b)[b->get_next_free_photon()]);
+          ++k;
+        }
+        work.push_back(b);
+      }
+      while (!work.empty()) {
+        PhotonBuffer *in = work.back();
+        work.pop_back();
+        DensitySubGrid &g = *grid.sub[in->get_subgrid_index()];
+        for (int d = 0; d < TRAVELDIRECTION_NUMBER; ++d) local[d].reset();
+        for (unsigned int i = 0; i < in->size(); ++i) {
+          Photon &p = (*in)[i];
+          const int out = g.interact(p, in->get_direction());
+          if (out != TRAVELDIRECTION_INSIDE &&
+              g.get_neighbour(out) != NEIGHBOUR_OUTSIDE) {
+            local[out][local[out].get_next_free_photon()] = p;
           }
         }
-        if (b == (size_t)-1 && queue.empty()) {
-          /* starving: flush the largest non-empty active buffer
-           * (testDensitySubGrid_omp.cpp:1503-) */
-          for (int sg = 0; sg < nsg && b == (size_t)-1; ++sg) {
-            if (!sublock[sg].try_lock()) continue;
-            DensitySubGrid &g = *grid.sub[sg];
-            for (int d = 1; d < TRAVELDIRECTION_NUMBER; ++d) {
-              const unsigned int ab = g.get_active_buffer(d);
-              if (ab != NEIGHBOUR_OUTSIDE && pool[ab].size() > 0) {
-                g.set_active_buffer(d, NEIGHBOUR_OUTSIDE);
-                queue.push_back(ab);
-                break;
-              }
-            }
-            sublock[sg].unlock();
-          }
-          continue;
+        spare.push_back(in);
+        for (int d = 1; d < TRAVELDIRECTION_NUMBER; ++d) {
+          if (local[d].size() == 0) continue;
+          PhotonBuffer *b = get_buffer();
+          *b = local[d];
+          b->set_subgrid_index(g.get_neighbour(d));
+          b->set_direction(TravelDirections::output_to_input_direction(d));
+          work.push_back(b);
         }
       }
-      if (b == (size_t)-1) continue;
-      PhotonBuffer &in = pool[b];
-      const unsigned int sg = in.get_subgrid_index();
-      DensitySubGrid &g = *grid.sub[sg];
-      for (int d = 0; d < TRAVELDIRECTION_NUMBER; ++d) local[d].reset();
-      size_t finished = 0;
-      for (unsigned int i = 0; i < in.size(); ++i) {
-        Photon &p = in[i];
-        const int out = g.interact(p, in.get_direction());
-        if (out == TRAVELDIRECTION_INSIDE ||
-            g.get_neighbour(out) == NEIGHBOUR_OUTSIDE) {
-          ++finished;
-        } else {
-          local[out][local[out].get_next_free_photon()] = p;
-        }
-      }
-      std::vector<size_t> ready;
-      for (int d = 1; d < TRAVELDIRECTION_NUMBER; ++d) {
-        if (local[d].size() == 0) continue;
-        unsigned int ab = g.get_active_buffer(d);
-        if (ab == NEIGHBOUR_OUTSIDE) {
-          ab = (unsigned int)pool.get_free_buffer();
-          pool[ab].reset();
-          pool[ab].set_subgrid_index(g.get_neighbour(d));
-          pool[ab].set_direction(TravelDirections::output_to_input_direction(d));
-          g.set_active_buffer(d, ab);
-        }
-        const size_t nb = pool.add_photons(ab, local[d]);
-        if (nb != ab) {
-          ready.push_back(ab);
-          if (pool[nb].size() > 0) {
-            g.set_active_buffer(d, (unsigned int)nb);
-          } else {
-            pool.free_buffer(nb);
-            g.set_active_buffer(d, NEIGHBOUR_OUTSIDE);
-          }
-        }
-      }
-      pool.free_buffer(b);
-      sublock[sg].unlock();
-      if (!ready.empty()) {
-        std::lock_guard<std::mutex> lk(qmutex);
-        queue.insert(queue.end(), ready.begin(), ready.end());
-      }
-      done += finished;
     }
+    for (PhotonBuffer *b : spare) delete b;
   }
   const auto t1 = std::chrono::steady_clock::now();
-  grid.get_J(J, sc[0], sc[1], sc[2]);
+  const size_t nc = grid_cells(ncell);
+  std::vector<double> Jt(nc);
+  for (size_t i = 0; i < nc; ++i) J[i] = 0.;
+  for (int t = 0; t < nthreads; ++t) {
+    grids[t]->get_J(Jt.data(), sc[0], sc[1], sc[2]);
+    for (size_t i = 0; i < nc; ++i) J[i] += Jt[i];
+    delete grids[t];
+  }
   return std::chrono::duration<double>(t1 - t0).count();
 }
 

@@ -12,7 +12,7 @@ import numpy as np
 
 __all__ = [
     "CMIError", "Context", "PHOTON_DTYPE", "load_library", "PREC_FP64",
-    "PREC_FP32", "ENGINE_WALK", "ENGINE_TILES", "INSIDE", "output_to_input",
+    "PREC_FP32", "ENGINE_WALK", "ENGINE_TILES", "INSIDE", "output_to_input", "Batch",
     "direction_from_mask", "STATS_FIELDS",
 ]
 
@@ -95,6 +95,16 @@ SIGNATURES = {
     "cmi_get_sizes": (C.c_int, [_vp, _u64p, _u64p]),
     "cmi_kernel_timing": (C.c_int, [_vp, _dp, _u64p, C.c_int]),
     "cmi_synchronize": (C.c_int, [_vp]),
+    "cmi_batch_create": (C.c_int, [C.c_int, _dp, _ip, C.POINTER(Tuning),
+                                   C.POINTER(_vp)]),
+    "cmi_batch_destroy": (None, [_vp]),
+    "cmi_batch_last_error": (C.c_char_p, [_vp]),
+    "cmi_batch_set_fields": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "cmi_batch_get_fields": (C.c_int, [_vp, C.c_int, _dp, _dp]),
+    "cmi_batch_set_sources": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int]),
+    "cmi_batch_iteration": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint32,
+                                      C.c_double, C.c_int, _dp, _dp, _u64p,
+                                      C.POINTER(Stats)]),
 }
 
 
@@ -285,6 +295,89 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.cmi_synchronize(self.h))
+
+
+class Batch:
+    """A problem set: many independent grids transported together
+    (cmi_batch_*).  Cell arrays of a problem are (nx, ny, nz) lattices."""
+
+    def __init__(self, boxes, ncells, tile=16, block_threads=0, refill_batch=0,
+                 chunk=0):
+        self.lib = load_library()
+        b = np.ascontiguousarray(boxes, dtype=np.float64).reshape(-1, 6)
+        nc = np.ascontiguousarray(ncells, dtype=np.int32).reshape(-1, 3)
+        if len(b) != len(nc):
+            raise ValueError("one box per problem")
+        t = Tuning()
+        t.tile[0] = t.tile[1] = t.tile[2] = int(tile)
+        t.block_threads = int(block_threads)
+        t.refill_batch = int(refill_batch)
+        t.chunk_min = int(chunk)
+        h = _vp()
+        rc = self.lib.cmi_batch_create(len(b), _p(b, _dp), _p(nc, _ip),
+                                       C.byref(t), C.byref(h))
+        if rc != 0:
+            raise CMIError(rc, "cmi_batch_create failed")
+        self.h = h
+        self.boxes = b
+        self.ncells = nc
+        self.nprob = len(b)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cmi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CMIError(rc, self.lib.cmi_batch_last_error(self.h).decode())
+
+    def set_fields(self, p, nH=None, xn=None):
+        a = None if nH is None else np.ascontiguousarray(nH, dtype=np.float64)
+        b = None if xn is None else np.ascontiguousarray(xn, dtype=np.float64)
+        self._check(self.lib.cmi_batch_set_fields(
+            self.h, int(p), None if a is None else _p(a, _dp),
+            None if b is None else _p(b, _dp)))
+
+    def get_fields(self, p):
+        n = int(np.prod(self.ncells[p]))
+        J = np.zeros(n)
+        x = np.zeros(n)
+        self._check(self.lib.cmi_batch_get_fields(self.h, int(p), _p(J, _dp),
+                                                  _p(x, _dp)))
+        shape = tuple(int(v) for v in self.ncells[p])
+        return J.reshape(shape), x.reshape(shape)
+
+    def set_sources(self, p, pos, L):
+        a = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        l = np.ascontiguousarray(L, dtype=np.float64).reshape(-1)
+        self._check(self.lib.cmi_batch_set_sources(self.h, int(p), _p(a, _dp),
+                                                   _p(l, _dp), len(l)))
+
+    def iteration(self, npackets, seed, iteration, alpha, form=1):
+        n = np.ascontiguousarray(npackets, dtype=np.uint64)
+        l1 = np.zeros(self.nprob)
+        secs = np.zeros(self.nprob)
+        cross = np.zeros(self.nprob, np.uint64)
+        s = Stats()
+        self._check(self.lib.cmi_batch_iteration(
+            self.h, _p(n, _u64p), int(seed), int(iteration), float(alpha),
+            int(form), _p(l1, _dp), _p(secs, _dp), _p(cross, _u64p),
+            C.byref(s)))
+        return {"l1": l1, "problem_seconds": secs, "problem_crossings": cross,
+                **_stats_dict(s)}
 
 
 def _stats_dict(s):

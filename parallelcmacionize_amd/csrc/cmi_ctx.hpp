@@ -2,6 +2,7 @@
 // shared by the C-ABI layer and the engines' host code.
 #pragma once
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,36 @@ inline int fail(cmi_ctx *ctx, int code, const std::string &msg) {
                          std::string(#call) + ": " + hipGetErrorString(e_)); \
     }                                                                       \
   } while (0)
+
+// testDensitySubGrid_omp.cpp:2199-2208, 2240-2273 (split over sources with a
+// random remainder, distribute() at 1650-1677): floor(L_i / L_tot * N) each,
+// the remainder one each to the sources with the smallest Philox keys.
+inline void source_counts(const std::vector<double> &L, uint64_t npackets,
+                   uint64_t seed, uint32_t iteration,
+                   std::vector<uint64_t> &counts) {
+  const int nsrc = (int)L.size();
+  double Ltot = 0.;
+  for (double v : L) Ltot += v;
+  counts.assign(nsrc, 0);
+  uint64_t added = 0;
+  for (int i = 0; i < nsrc; ++i) {
+    counts[i] = (uint64_t)(L[i] / Ltot * (double)npackets);
+    added += counts[i];
+  }
+  const uint64_t rem = npackets - added;
+  if (rem == 0) return;
+  std::vector<double> key(nsrc);
+  std::vector<int> order(nsrc);
+  for (int i = 0; i < nsrc; ++i) {
+    const u32x4 o = philox4x32_10(u32x4{(uint32_t)i, 0u, iteration, 0x5EEDu},
+                                  (uint32_t)seed, (uint32_t)(seed >> 32));
+    key[i] = uniform_open(o.x, o.y);
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return key[a] < key[b]; });
+  for (uint64_t r = 0; r < rem && r < (uint64_t)nsrc; ++r) counts[order[r]]++;
+}
 
 // Tiles engine (cmi_tiles.hip).
 int tiles_transport_packets(cmi_ctx *c, cmi_photon *p,

@@ -27,6 +27,7 @@
 // together, so the divergent refill path is paid once per batch rather than
 // once per packet.
 #include <float.h>
+#include <string.h>
 
 #include "cmi_ctx.hpp"
 
@@ -34,49 +35,83 @@ namespace cmi {
 
 constexpr uint32_t kNone = 0xffffffffu;
 
-// Tile lattice of one grid: the global tile (tx, ty, tz) of a cell is
-// (gx >> tshift, ...); its device index is subgrid-major then tile-in-subgrid
-// (the order cell_addr() uses), so tile * tile_cells is its first cell.
-struct TileGeo {
-  int N[3];         // cells per axis (global)
-  int T[3];         // tile size per axis
-  int tshift[3];
-  int nts[3];       // tiles per subgrid per axis
-  int nsub[3];
-  int ntl[3];       // tiles per axis (global)
+// A problem is one grid of subgrids; a problem set is one or more problems
+// that share the LDS tile shape and are transported together.  Tiles of all
+// problems form one list: problem p owns tiles [tile_base, tile_base +
+// ntiles_p), and inside a problem the order is subgrid-major then
+// tile-in-subgrid (the order cell_addr() uses), so tile * tile_cells is the
+// tile's first cell in the concatenated cell arrays.
+struct ProbGeo {
+  int N[3];          // cells per axis
+  int nts[3];        // tiles per subgrid per axis
+  int nsub[3];       // subgrids per axis
+  int ntl[3];        // tiles per axis
   int tiles_per_sub;
-  int tile_cells;
-  int ntiles;
-  int pad_;
-  double box[3];
+  int tile_base;     // first tile of this problem in the tile list
+  double box[3];     // anchor of the box (m)
   double inv_cell_size[3];
-  double unit_scale[3];
+  double cell_size[3];
+  double unit_scale[3];  // Delta_0 / Delta_a
 };
 
-__device__ __forceinline__ int tile_index(const TileGeo &G, int tx, int ty,
+struct TileGeo {
+  int T[3];          // tile size per axis (cells, powers of two)
+  int tshift[3];
+  int tile_cells;
+  int ntiles;        // tiles of all problems
+  int nprob;
+  int pad_;
+  const ProbGeo *probs;       // [nprob]
+  const uint16_t *tile_prob;  // [ntiles] problem of each tile
+};
+
+__device__ __forceinline__ int tile_index(const ProbGeo &P, int tx, int ty,
                                           int tz) {
-  const int sx = tx / G.nts[0], sy = ty / G.nts[1], sz = tz / G.nts[2];
-  const int sub = (sx * G.nsub[1] + sy) * G.nsub[2] + sz;
-  const int t = ((tx - sx * G.nts[0]) * G.nts[1] + (ty - sy * G.nts[1])) *
-                    G.nts[2] +
-                (tz - sz * G.nts[2]);
-  return sub * G.tiles_per_sub + t;
+  const int sx = tx / P.nts[0], sy = ty / P.nts[1], sz = tz / P.nts[2];
+  const int sub = (sx * P.nsub[1] + sy) * P.nsub[2] + sz;
+  const int t = ((tx - sx * P.nts[0]) * P.nts[1] + (ty - sy * P.nts[1])) *
+                    P.nts[2] +
+                (tz - sz * P.nts[2]);
+  return P.tile_base + sub * P.tiles_per_sub + t;
 }
 
-__device__ __forceinline__ void tile_coords(const TileGeo &G, int tile, int &tx,
-                                            int &ty, int &tz) {
-  const int sub = tile / G.tiles_per_sub;
-  const int t = tile - sub * G.tiles_per_sub;
-  const int sx = sub / (G.nsub[1] * G.nsub[2]);
-  const int sy = (sub / G.nsub[2]) % G.nsub[1];
-  const int sz = sub % G.nsub[2];
-  const int lx = t / (G.nts[1] * G.nts[2]);
-  const int ly = (t / G.nts[2]) % G.nts[1];
-  const int lz = t % G.nts[2];
-  tx = sx * G.nts[0] + lx;
-  ty = sy * G.nts[1] + ly;
-  tz = sz * G.nts[2] + lz;
+__device__ __forceinline__ void tile_coords(const ProbGeo &P, int tile,
+                                            int &tx, int &ty, int &tz) {
+  const int local = tile - P.tile_base;
+  const int sub = local / P.tiles_per_sub;
+  const int t = local - sub * P.tiles_per_sub;
+  const int sx = sub / (P.nsub[1] * P.nsub[2]);
+  const int sy = (sub / P.nsub[2]) % P.nsub[1];
+  const int sz = sub % P.nsub[2];
+  const int lx = t / (P.nts[1] * P.nts[2]);
+  const int ly = (t / P.nts[2]) % P.nts[1];
+  const int lz = t % P.nts[2];
+  tx = sx * P.nts[0] + lx;
+  ty = sy * P.nts[1] + ly;
+  tz = sz * P.nts[2] + lz;
 }
+
+// Sources of a problem set: problem p emits packets [pk_cum[p-1], pk_cum[p])
+// of the call, from its sources src_begin[p] .. src_begin[p+1]-1 (packet
+// numbers within the problem split by src_cum, cumulative per problem).
+constexpr int kMaxProblems = 256;
+constexpr int kMaxSetSources = 4096;
+struct SetSources {
+  int nprob;
+  uint32_t iteration;
+  uint64_t seed;
+  uint64_t pk_cum[kMaxProblems];
+  int src_begin[kMaxProblems + 1];
+  uint64_t src_cum[kMaxSetSources];
+  double pos[kMaxSetSources][3];
+};
+
+// Per-problem ionization constants.
+struct IonParams {
+  double jfac;         // L / N / V_cell
+  double jscale;       // J units -> m^3 (sigma Delta_0 in FP32 mode)
+  double kappa_scale;  // sigma Delta_0
+};
 
 // FP32 packet record, 32 bytes: position relative to the tile origin in cell
 // units, direction scaled to cells per Delta_0, remaining optical depth, and
@@ -116,33 +151,68 @@ struct PacketOut {
 // ---------------------------------------------------------------------------
 // Emission straight into FP32 records (round-0 input).
 // ---------------------------------------------------------------------------
-__global__ void tiles_emit_kernel(TileGeo G, const SourcesDev *__restrict__ S,
-                                  uint64_t npackets, uint32_t iteration,
-                                  uint32_t key0, uint32_t key1,
-                                  Rec32 *__restrict__ out,
+// Packet k of problem p draws counters {k_lo, k_hi ^ (p << 20), iteration, 0}
+// and {..., 1} (problem 0 draws exactly what emit_packet() draws).
+__device__ __forceinline__ int set_emit(const SetSources &S, uint64_t kg,
+                                        uint32_t iteration, uint32_t key0,
+                                        uint32_t key1, double pos[3],
+                                        double dir[3], double *tau) {
+  int p = 0;
+  while (p + 1 < S.nprob && kg >= S.pk_cum[p]) ++p;
+  const uint64_t k = kg - (p ? S.pk_cum[p - 1] : 0);
+  int s = S.src_begin[p];
+  const int s_end = S.src_begin[p + 1];
+  while (s + 1 < s_end && k >= S.src_cum[s]) ++s;
+  const u32x4 r0 = philox4x32_10(
+      u32x4{(uint32_t)k, (uint32_t)(k >> 32) ^ ((uint32_t)p << 20), iteration, 0u},
+      key0, key1);
+  const u32x4 r1 = philox4x32_10(
+      u32x4{(uint32_t)k, (uint32_t)(k >> 32) ^ ((uint32_t)p << 20), iteration, 1u},
+      key0, key1);
+  const double cost = 2. * uniform_open(r0.x, r0.y) - 1.;
+  const double phi = 2. * 3.14159265358979323846 * uniform_open(r0.z, r0.w);
+  const double sint = sqrt(fmax(1. - cost * cost, 0.));
+  double sp, cp;
+  sincos(phi, &sp, &cp);
+  dir[0] = sint * cp;
+  dir[1] = sint * sp;
+  dir[2] = cost;
+  *tau = -log(uniform_open(r1.x, r1.y));
+  pos[0] = S.pos[s][0];
+  pos[1] = S.pos[s][1];
+  pos[2] = S.pos[s][2];
+  return p;
+}
+
+// Seed and iteration come from S (device memory), so that a captured graph
+// of the iteration can be replayed with new values.
+__global__ void tiles_emit_kernel(TileGeo G, const SetSources *__restrict__ S,
+                                  uint64_t npackets, Rec32 *__restrict__ out,
                                   uint32_t *__restrict__ dest) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < npackets) {
     double pos[3], dir[3], tau;
-    emit_packet(*S, k, iteration, key0, key1, pos, dir, &tau);
+    const int p = set_emit(*S, k, S->iteration, (uint32_t)S->seed,
+                           (uint32_t)(S->seed >> 32), pos, dir, &tau);
+    const ProbGeo &P = G.probs[p];
     int g[3];
     double pc[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      pc[a] = (pos[a] - G.box[a]) * G.inv_cell_size[a];
+      pc[a] = (pos[a] - P.box[a]) * P.inv_cell_size[a];
       int c = (int)pc[a];
-      g[a] = c < 0 ? 0 : (c >= G.N[a] ? G.N[a] - 1 : c);
+      g[a] = c < 0 ? 0 : (c >= P.N[a] ? P.N[a] - 1 : c);
     }
     const int tx = g[0] >> G.tshift[0], ty = g[1] >> G.tshift[1],
               tz = g[2] >> G.tshift[2];
-    const uint32_t tile = (uint32_t)tile_index(G, tx, ty, tz);
+    const uint32_t tile = (uint32_t)tile_index(P, tx, ty, tz);
     Rec32 r;
     r.a.x = (float)(pc[0] - (double)(tx << G.tshift[0]));
     r.a.y = (float)(pc[1] - (double)(ty << G.tshift[1]));
     r.a.z = (float)(pc[2] - (double)(tz << G.tshift[2]));
-    r.a.w = (float)(dir[0] * G.unit_scale[0]);
-    r.b.x = (float)(dir[1] * G.unit_scale[1]);
-    r.b.y = (float)(dir[2] * G.unit_scale[2]);
+    r.a.w = (float)(dir[0] * P.unit_scale[0]);
+    r.b.x = (float)(dir[1] * P.unit_scale[1]);
+    r.b.y = (float)(dir[2] * P.unit_scale[2]);
     r.b.z = (float)tau;
     r.b.w = __uint_as_float(pack_cell(g[0], g[1], g[2]));
     out[k] = r;
@@ -153,22 +223,26 @@ __global__ void tiles_emit_kernel(TileGeo G, const SourcesDev *__restrict__ S,
 // Tile of every source's start cell (the cell the emission kernel files its
 // packets under) gets that source's packet count.
 __global__ void tiles_source_hist_kernel(TileGeo G,
-                                         const SourcesDev *__restrict__ S,
+                                         const SetSources *__restrict__ S,
                                          uint32_t *__restrict__ hist) {
-  const int s = threadIdx.x;
-  if (s >= S->nsrc) return;
-  const uint64_t c = S->cum[s] - (s ? S->cum[s - 1] : 0);
-  if (c == 0) return;
-  int g[3];
+  for (int p = 0; p < S->nprob; ++p) {
+    const ProbGeo &P = G.probs[p];
+    const int b = S->src_begin[p], e = S->src_begin[p + 1];
+    for (int s = b + (int)threadIdx.x; s < e; s += blockDim.x) {
+      const uint64_t c = S->src_cum[s] - (s > b ? S->src_cum[s - 1] : 0);
+      if (c == 0) continue;
+      int g[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const double pc = (S->pos[s][a] - G.box[a]) * G.inv_cell_size[a];
-    const int cc = (int)pc;
-    g[a] = cc < 0 ? 0 : (cc >= G.N[a] ? G.N[a] - 1 : cc);
+      for (int a = 0; a < 3; ++a) {
+        const double pc = (S->pos[s][a] - P.box[a]) * P.inv_cell_size[a];
+        const int cc = (int)pc;
+        g[a] = cc < 0 ? 0 : (cc >= P.N[a] ? P.N[a] - 1 : cc);
+      }
+      const int tile = tile_index(P, g[0] >> G.tshift[0], g[1] >> G.tshift[1],
+                                  g[2] >> G.tshift[2]);
+      atomicAdd(&hist[tile], (uint32_t)c);
+    }
   }
-  const int tile = tile_index(G, g[0] >> G.tshift[0], g[1] >> G.tshift[1],
-                              g[2] >> G.tshift[2]);
-  atomicAdd(&hist[tile], (uint32_t)c);
 }
 
 // Explicit FP64 packet list -> FP32 records (start cell found like the
@@ -183,10 +257,11 @@ __global__ void tiles_load_list_kernel(TileGeo G, GridDev g,
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const PhotonRec ph = p[i];
+  const ProbGeo &P = G.probs[0];
   const int sub = (int)start_sub[i];
-  const int sx = sub / (G.nsub[1] * G.nsub[2]);
-  const int sy = (sub / G.nsub[2]) % G.nsub[1];
-  const int sz = sub % G.nsub[2];
+  const int sx = sub / (P.nsub[1] * P.nsub[2]);
+  const int sy = (sub / P.nsub[2]) % P.nsub[1];
+  const int sz = sub % P.nsub[2];
   const int so[3] = {sx * g.sc[0], sy * g.sc[1], sz * g.sc[2]};
   const double *an = g.anchors + 3 * (size_t)sub;
   int gc[3];
@@ -201,14 +276,14 @@ __global__ void tiles_load_list_kernel(TileGeo G, GridDev g,
   }
   const int tx = gc[0] >> G.tshift[0], ty = gc[1] >> G.tshift[1],
             tz = gc[2] >> G.tshift[2];
-  const uint32_t tile = (uint32_t)tile_index(G, tx, ty, tz);
+  const uint32_t tile = (uint32_t)tile_index(P, tx, ty, tz);
   Rec32 r;
   r.a.x = (float)(pc[0] - (double)(tx << G.tshift[0]));
   r.a.y = (float)(pc[1] - (double)(ty << G.tshift[1]));
   r.a.z = (float)(pc[2] - (double)(tz << G.tshift[2]));
-  r.a.w = (float)(ph.dir[0] * G.unit_scale[0]);
-  r.b.x = (float)(ph.dir[1] * G.unit_scale[1]);
-  r.b.y = (float)(ph.dir[2] * G.unit_scale[2]);
+  r.a.w = (float)(ph.dir[0] * P.unit_scale[0]);
+  r.b.x = (float)(ph.dir[1] * P.unit_scale[1]);
+  r.b.y = (float)(ph.dir[2] * P.unit_scale[2]);
   r.b.z = (float)ph.tau;
   r.b.w = __uint_as_float(pack_cell(gc[0], gc[1], gc[2]));
   out[i] = r;
@@ -255,8 +330,19 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     int ntiles, uint32_t *__restrict__ hist, uint32_t *__restrict__ seg,
     uint32_t *__restrict__ cursor, Desc *__restrict__ desc,
     uint32_t *__restrict__ ndesc, uint32_t *__restrict__ total,
-    unsigned *__restrict__ ticket, uint32_t chunk) {
+    unsigned *__restrict__ ticket, uint32_t chunk,
+    const uint16_t *__restrict__ tile_prob, int nprob,
+    uint32_t *__restrict__ prob_round,
+    unsigned long long *__restrict__ prob_time,
+    uint32_t *__restrict__ round_ctr) {
   __shared__ uint64_t sh[17];
+  __shared__ uint32_t s_pc[kMaxProblems];
+  __shared__ uint32_t s_round;
+  const bool track = prob_round != nullptr;
+  if (track) {
+    for (int q = threadIdx.x; q < nprob; q += kPlanThreads) s_pc[q] = 0;
+    __syncthreads();
+  }
   const int per = (ntiles + kPlanThreads - 1) / kPlanThreads;
   const int t0 = threadIdx.x * per;
   const int t1 = min(ntiles, t0 + per);
@@ -265,6 +351,7 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     const uint32_t c = hist[t];
     cnt += c;
     nch += (c + chunk - 1) / chunk;
+    if (track && c) atomicAdd(&s_pc[tile_prob[t]], c);
   }
   uint64_t tot_c, tot_d;
   uint64_t base_c = block_exclusive_scan(cnt, sh, &tot_c);
@@ -291,6 +378,43 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     *total = (uint32_t)tot_c;
     *ticket = 0;
   }
+  if (track) {
+    // completion of each problem: the first plan that finds it empty
+    if (threadIdx.x == 0) {
+      s_round = *round_ctr;
+      *round_ctr = s_round + 1;
+    }
+    __syncthreads();
+    const unsigned long long now = wall_clock64();
+    for (int q = threadIdx.x; q < nprob; q += kPlanThreads) {
+      if (s_pc[q] == 0 && prob_round[q] == kNone) {
+        prob_round[q] = s_round;
+        prob_time[q] = now;
+      }
+    }
+  }
+}
+
+// Start of a set iteration: per-problem completion marks cleared, the round
+// counter zeroed and the start time taken.
+__global__ void tiles_begin_kernel(int nprob, uint32_t *__restrict__ prob_round,
+                                   unsigned long long *__restrict__ prob_time,
+                                   uint32_t *__restrict__ round_ctr,
+                                   unsigned long long *__restrict__ t0) {
+  for (int q = threadIdx.x; q < nprob; q += blockDim.x) {
+    prob_round[q] = kNone;
+    prob_time[q] = 0;
+  }
+  if (threadIdx.x == 0) {
+    *round_ctr = 0;
+    *t0 = wall_clock64();
+  }
+}
+
+// total[1] = total[0] (kept for the scatter), as a kernel so that the round
+// loop can be captured in a graph without memcpy nodes.
+__global__ void copy_total_kernel(uint32_t *__restrict__ total) {
+  if (threadIdx.x == 0) total[1] = total[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -385,7 +509,8 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     const uint32_t *__restrict__ idx, Rec32 *__restrict__ Pout,
     uint32_t *__restrict__ dest_out, uint32_t *__restrict__ hist_next,
     const uint32_t *__restrict__ pid_in, uint32_t *__restrict__ pid_out,
-    PacketOut po, Counters *__restrict__ cnt, int refill_batch) {
+    PacketOut po, Counters *__restrict__ cnt,
+    unsigned long long *__restrict__ prob_cross, int refill_batch) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *kapL = reinterpret_cast<float *>(smem);
   Acc *accL = reinterpret_cast<Acc *>(smem + sizeof(float) * G.tile_cells);
@@ -408,6 +533,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     if (di >= ndesc) break;
     const Desc D = desc[di];
     const int tile = D.tile;
+    const int prob = G.tile_prob[tile];
+    const ProbGeo &P = G.probs[prob];
+    const unsigned long long cross0 = my_cross;
     // stage the tile
     {
       const float4 *src =
@@ -423,7 +551,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       if (threadIdx.x < 6) s_face[threadIdx.x] = 0;
     }
     int ttx, tty, ttz;
-    tile_coords(G, tile, ttx, tty, ttz);
+    tile_coords(P, tile, ttx, tty, ttz);
     const int ox = ttx << G.tshift[0], oy = tty << G.tshift[1],
               oz = ttz << G.tshift[2];
 
@@ -495,7 +623,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             r.b.w = __uint_as_float(pack_cell(gx, gy, gz));
             const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
             Pout[slot] = r;
-            dest_out[slot] = (uint32_t)tile_index(G, ntx, nty, ntz);
+            dest_out[slot] = (uint32_t)tile_index(P, ntx, nty, ntz);
             if (kList) pid_out[slot] = my_pid;
             const float de = exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2);
             atomicAdd(&s_face[2 * exit_axis + (de > 0.f ? 0 : 1)], 1u);
@@ -603,9 +731,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       if (out0 | out1 | out2) {
         exit_axis = out0 ? 0 : (out1 ? 1 : 2);
         const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
-        const bool in_box = (unsigned)gx < (unsigned)G.N[0] &&
-                            (unsigned)gy < (unsigned)G.N[1] &&
-                            (unsigned)gz < (unsigned)G.N[2];
+        const bool in_box = (unsigned)gx < (unsigned)P.N[0] &&
+                            (unsigned)gy < (unsigned)P.N[1] &&
+                            (unsigned)gz < (unsigned)P.N[2];
         outcome = in_box ? 1 : 2;
         live = false;
         pending = true;
@@ -631,11 +759,18 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         if (c) {
           const int ax = threadIdx.x >> 1;
           const int sg = (threadIdx.x & 1) ? -1 : 1;
-          const int nt = tile_index(G, ttx + (ax == 0 ? sg : 0),
+          const int nt = tile_index(P, ttx + (ax == 0 ? sg : 0),
                                     tty + (ax == 1 ? sg : 0),
                                     ttz + (ax == 2 ? sg : 0));
           atomicAdd(&hist_next[nt], c);
         }
+      }
+      if (prob_cross) {
+        // crossings of this visit, per problem (one atomic per wave)
+        unsigned long long v = my_cross - cross0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0 && v) atomicAdd(&prob_cross[prob], v);
       }
     }
     __syncthreads();
@@ -679,16 +814,66 @@ __global__ void tiles_finish_list_kernel(TileGeo G, GridDev g, size_t n,
   if (i >= n) return;
   const int tile = ftile[i];
   if (tile < 0) return;
+  const ProbGeo &P = G.probs[0];
   int tx, ty, tz;
-  tile_coords(G, tile, tx, ty, tz);
+  tile_coords(P, tile, tx, ty, tz);
   const float4 a = fa[i];
   const float4 b = fb[i];
   const int o[3] = {tx << G.tshift[0], ty << G.tshift[1], tz << G.tshift[2]};
   const float q[3] = {a.x, a.y, a.z};
 #pragma unroll
   for (int k = 0; k < 3; ++k)
-    p[i].pos[k] = G.box[k] + ((double)o[k] + (double)q[k]) * g.cell_size[k];
+    p[i].pos[k] = P.box[k] + ((double)o[k] + (double)q[k]) * g.cell_size[k];
   p[i].tau = (double)b.z;
+}
+
+// ---------------------------------------------------------------------------
+// Per-problem ionization update of a problem set (FP64): one block per tile,
+// the tile's cells belong to one problem; the L1 change is summed per problem.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) set_ionize_kernel(
+    TileGeo G, const IonParams *__restrict__ ip, const double *__restrict__ n,
+    double *__restrict__ xn, double *__restrict__ J, float *__restrict__ kappa,
+    double alpha, int form, double *__restrict__ l1) {
+#pragma clang fp contract(off)
+  __shared__ double sh[4];
+  for (int tile = blockIdx.x; tile < G.ntiles; tile += gridDim.x) {
+    const int p = G.tile_prob[tile];
+    const IonParams P = ip[p];
+    double acc = 0.;
+    const size_t base = (size_t)tile * G.tile_cells;
+    for (int k = threadIdx.x; k < G.tile_cells; k += blockDim.x) {
+      const size_t i = base + k;
+      const double jH = P.jfac * (J[i] * P.jscale);
+      const double ntot = n[i];
+      double x;
+      if (jH > 0.) {
+        const double aa = 0.5 * jH / (ntot * alpha);
+        if (form == 0) {
+          x = 1. + aa * (1. - sqrt(2. / aa + 1.));
+        } else {
+          const double b = 2. * aa;
+          x = 2. / (2. + b + sqrt(b * (b + 4.)));
+        }
+      } else {
+        x = 1.;
+      }
+      acc += fabs(x - xn[i]);
+      xn[i] = x;
+      J[i] = 0.;
+      kappa[i] = (float)(P.kappa_scale * ntot * x);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += sh[w];
+      if (s != 0.) atomicAdd(&l1[p], s);
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -697,6 +882,7 @@ __global__ void tiles_finish_list_kernel(TileGeo G, GridDev g, size_t n,
 struct TilesEngine {
   size_t cap = 0;         // packet capacity
   int ntiles = 0;
+  int nprob = 0;
   size_t desc_cap = 0;
   Rec32 *P[2] = {nullptr, nullptr};
   uint32_t *dest[2] = {nullptr, nullptr};
@@ -710,10 +896,26 @@ struct TilesEngine {
   uint32_t *h_total = nullptr;  // pinned
   int tile_key[3] = {0, 0, 0};
   std::vector<hipEvent_t> ev;   // start/stop of every process launch
+  // problem set
+  ProbGeo *d_probs = nullptr;
+  uint16_t *d_tile_prob = nullptr;
+  SetSources *d_src = nullptr, *h_src = nullptr;  // device / pinned host
+  IonParams *d_ion = nullptr, *h_ion = nullptr;
+  double *d_l1 = nullptr;
+  uint32_t *d_prob_round = nullptr, *d_round_ctr = nullptr;
+  unsigned long long *d_prob_time = nullptr, *d_t0 = nullptr;
+  unsigned long long *d_prob_cross = nullptr;
+  std::vector<ProbGeo> probs;
+  // captured iteration graph of a problem set
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t graph_key = 0;
 };
 
 static void free_engine(TilesEngine *e) {
   if (!e) return;
+  if (e->graph_exec) hipGraphExecDestroy(e->graph_exec);
+  if (e->graph) hipGraphDestroy(e->graph);
   for (int i = 0; i < 2; ++i) {
     hipFree(e->P[i]);
     hipFree(e->dest[i]);
@@ -728,6 +930,18 @@ static void free_engine(TilesEngine *e) {
   hipFree(e->total);
   hipFree(e->ticket);
   if (e->h_total) hipHostFree(e->h_total);
+  hipFree(e->d_probs);
+  hipFree(e->d_tile_prob);
+  hipFree(e->d_src);
+  if (e->h_src) hipHostFree(e->h_src);
+  hipFree(e->d_ion);
+  if (e->h_ion) hipHostFree(e->h_ion);
+  hipFree(e->d_l1);
+  hipFree(e->d_prob_round);
+  hipFree(e->d_round_ctr);
+  hipFree(e->d_prob_time);
+  hipFree(e->d_t0);
+  hipFree(e->d_prob_cross);
   for (hipEvent_t v : e->ev) hipEventDestroy(v);
   delete e;
 }
@@ -737,43 +951,77 @@ void tiles_destroy(cmi_ctx *c) {
   c->tiles = nullptr;
 }
 
-static TileGeo make_geo(const cmi_ctx *c) {
-  TileGeo G{};
-  for (int a = 0; a < 3; ++a) {
-    G.N[a] = c->ncell[a];
-    G.T[a] = c->tile[a];
-    G.tshift[a] = c->g.tshift[a];
-    G.nts[a] = c->g.ntile[a];
-    G.nsub[a] = c->nsub[a];
-    G.ntl[a] = c->ncell[a] / c->tile[a];
-    G.box[a] = c->box[a];
-    G.inv_cell_size[a] = c->g.inv_cell_size[a];
-    G.unit_scale[a] = c->g.unit_scale[a];
-  }
-  G.tiles_per_sub = G.nts[0] * G.nts[1] * G.nts[2];
-  G.tile_cells = c->g.tile_cells;
-  G.ntiles = G.tiles_per_sub * (int)c->nsubgrid;
-  return G;
-}
-
 static const uint32_t kChunk = 8192;      // packets per workgroup visit
 static const uint32_t kChunkFloor = 256;  // smallest chunk accepted
 
-static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
-  TilesEngine *e = c->tiles;
-  const int ntiles = make_geo(c).ntiles;
-  if (e && (e->cap < cap || e->ntiles != ntiles || (with_pid && !e->pid[0]) ||
-            e->tile_key[0] != c->tile[0] || e->tile_key[1] != c->tile[1] ||
-            e->tile_key[2] != c->tile[2])) {
-    free_engine(e);
-    c->tiles = e = nullptr;
+// The problem geometry of a single context: one problem, its own subgrids.
+static std::vector<ProbGeo> ctx_problems(const cmi_ctx *c) {
+  ProbGeo P{};
+  for (int a = 0; a < 3; ++a) {
+    P.N[a] = c->ncell[a];
+    P.nts[a] = c->g.ntile[a];
+    P.nsub[a] = c->nsub[a];
+    P.ntl[a] = c->ncell[a] / c->tile[a];
+    P.box[a] = c->box[a];
+    P.inv_cell_size[a] = c->g.inv_cell_size[a];
+    P.cell_size[a] = c->g.cell_size[a];
+    P.unit_scale[a] = c->g.unit_scale[a];
   }
-  if (e) return CMI_OK;
+  P.tiles_per_sub = P.nts[0] * P.nts[1] * P.nts[2];
+  P.tile_base = 0;
+  return {P};
+}
+
+static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
+  TileGeo G{};
+  for (int a = 0; a < 3; ++a) {
+    G.T[a] = c->tile[a];
+    G.tshift[a] = c->g.tshift[a];
+  }
+  G.tile_cells = c->g.tile_cells;
+  G.ntiles = e->ntiles;
+  G.nprob = e->nprob;
+  G.probs = e->d_probs;
+  G.tile_prob = e->d_tile_prob;
+  return G;
+}
+
+static int max_rounds_of(const std::vector<ProbGeo> &probs) {
+  // a packet's tile index moves monotonically along every axis, so it visits
+  // at most ntl_x + ntl_y + ntl_z - 2 tiles
+  int m = 0;
+  for (const ProbGeo &P : probs) m = std::max(m, P.ntl[0] + P.ntl[1] + P.ntl[2]);
+  return m + 2;
+}
+
+// (Re)build the engine for a problem set and a packet capacity.
+static int ensure_engine(cmi_ctx *c, const std::vector<ProbGeo> &probs,
+                         size_t cap, bool with_pid) {
+  TilesEngine *e = c->tiles;
+  int ntiles = 0;
+  for (const ProbGeo &P : probs)
+    ntiles += P.tiles_per_sub * P.nsub[0] * P.nsub[1] * P.nsub[2];
+  bool same = e && e->cap >= cap && e->ntiles == ntiles &&
+              e->nprob == (int)probs.size() && (!with_pid || e->pid[0]) &&
+              e->tile_key[0] == c->tile[0] && e->tile_key[1] == c->tile[1] &&
+              e->tile_key[2] == c->tile[2];
+  if (same) {
+    for (size_t q = 0; q < probs.size() && same; ++q)
+      same = memcmp(&e->probs[q], &probs[q], sizeof(ProbGeo)) == 0;
+  }
+  if (same) return CMI_OK;
+  free_engine(e);
+  c->tiles = e = nullptr;
+  if (probs.size() > (size_t)kMaxProblems)
+    return fail(c, CMI_ERR_ARG, "too many problems in one set");
   e = new TilesEngine();
   e->cap = cap;
   e->ntiles = ntiles;
+  e->nprob = (int)probs.size();
+  e->probs = probs;
   e->desc_cap = (size_t)ntiles + cap / kChunkFloor + 16;
   for (int a = 0; a < 3; ++a) e->tile_key[a] = c->tile[a];
+  const int np = e->nprob;
   hipError_t err = hipSuccess;
   for (int i = 0; i < 2 && err == hipSuccess; ++i) {
     err = hipMalloc(&e->P[i], cap * sizeof(Rec32));
@@ -789,14 +1037,38 @@ static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
   if (err == hipSuccess) err = hipMalloc(&e->total, 2 * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->ticket, sizeof(unsigned));
   if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 2 * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->d_probs, np * sizeof(ProbGeo));
+  if (err == hipSuccess) err = hipMalloc(&e->d_tile_prob, ntiles * sizeof(uint16_t));
+  if (err == hipSuccess) err = hipMalloc(&e->d_src, sizeof(SetSources));
+  if (err == hipSuccess) err = hipHostMalloc(&e->h_src, sizeof(SetSources));
+  if (err == hipSuccess) err = hipMalloc(&e->d_ion, np * sizeof(IonParams));
+  if (err == hipSuccess) err = hipHostMalloc(&e->h_ion, np * sizeof(IonParams));
+  if (err == hipSuccess) err = hipMalloc(&e->d_l1, np * sizeof(double));
+  if (err == hipSuccess) err = hipMalloc(&e->d_prob_round, np * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->d_round_ctr, sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->d_prob_time, np * sizeof(unsigned long long));
+  if (err == hipSuccess) err = hipMalloc(&e->d_t0, sizeof(unsigned long long));
+  if (err == hipSuccess) err = hipMalloc(&e->d_prob_cross, np * sizeof(unsigned long long));
   if (err != hipSuccess) {
     free_engine(e);
     return fail(c, CMI_ERR_ALLOC, "tiles engine: device allocation failed");
   }
+  std::vector<uint16_t> tp(ntiles);
+  for (int q = 0; q < np; ++q) {
+    const ProbGeo &P = probs[q];
+    const int nt = P.tiles_per_sub * P.nsub[0] * P.nsub[1] * P.nsub[2];
+    for (int t = 0; t < nt; ++t) tp[P.tile_base + t] = (uint16_t)q;
+  }
+  err = hipMemcpy(e->d_probs, probs.data(), np * sizeof(ProbGeo), hipMemcpyHostToDevice);
+  if (err == hipSuccess)
+    err = hipMemcpy(e->d_tile_prob, tp.data(), ntiles * sizeof(uint16_t), hipMemcpyHostToDevice);
+  if (err != hipSuccess) {
+    free_engine(e);
+    return fail(c, CMI_ERR_HIP, "tiles engine: set-up copy failed");
+  }
   hipLaunchKernelGGL(zero_u32_kernel, dim3(64), dim3(256), 0, c->stream, e->hist,
                      (size_t)ntiles);
-  const TileGeo G = make_geo(c);
-  const int max_rounds = 4 * (G.ntl[0] + G.ntl[1] + G.ntl[2]) + 16;
+  const int max_rounds = max_rounds_of(probs) + 2;
   e->ev.resize(2 * (size_t)max_rounds, nullptr);
   for (hipEvent_t &v : e->ev) {
     if (hipEventCreate(&v) != hipSuccess) {
@@ -808,67 +1080,90 @@ static int ensure_engine(cmi_ctx *c, size_t cap, bool with_pid) {
   return CMI_OK;
 }
 
-// Rounds of process / plan / scatter until no packet is left.  Round 0's
-// input (records, destinations, histogram) is already in P[0], dest[0],
-// hist; n0 packets.
+// Kernel-launch parameters of the engine.
+struct RoundCfg {
+  int threads, refill, grid;
+  uint32_t chunk;
+  bool fp32acc;
+  size_t lds;
+  unsigned scatter_blocks;
+};
+
+static RoundCfg round_cfg(const cmi_ctx *c, const TileGeo &G, size_t n0) {
+  RoundCfg r;
+  r.threads = c->tuning.block_threads == 256 ? 256 : 512;
+  r.refill = c->tuning.refill_batch > 0 ? std::min(64, c->tuning.refill_batch) : 32;
+  r.chunk = c->tuning.chunk_min > 0
+                ? std::max<uint32_t>(kChunkFloor, (uint32_t)c->tuning.chunk_min)
+                : kChunk;
+  r.fp32acc = c->tuning.lds_fp32 != 0;
+  r.lds = (size_t)G.tile_cells * (sizeof(float) + (r.fp32acc ? sizeof(float) : sizeof(double)));
+  const int wg_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (r.lds + 1024)));
+  r.grid = c->num_cus * wg_per_cu;
+  const size_t per_block = (size_t)kScatterThreads * kScatterItems;
+  r.scatter_blocks = (unsigned)std::min<size_t>(8192, (n0 + per_block - 1) / per_block + 1);
+  return r;
+}
+
+static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
+                           bool with_pid, PacketOut po,
+                           unsigned long long *prob_cross) {
+  TilesEngine *e = c->tiles;
+#define CMI_PROC(ACC, TH, LIST)                                                  \
+  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST>), dim3(R.grid), dim3(TH), \
+                     R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc, \
+                     e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,   \
+                     LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po, \
+                     c->d_cnt, prob_cross, R.refill)
+  if (with_pid) {
+    if (R.fp32acc) {
+      if (R.threads == 256) CMI_PROC(float, 256, true); else CMI_PROC(float, 512, true);
+    } else {
+      if (R.threads == 256) CMI_PROC(double, 256, true); else CMI_PROC(double, 512, true);
+    }
+  } else {
+    if (R.fp32acc) {
+      if (R.threads == 256) CMI_PROC(float, 256, false); else CMI_PROC(float, 512, false);
+    } else {
+      if (R.threads == 256) CMI_PROC(double, 256, false); else CMI_PROC(double, 512, false);
+    }
+  }
+#undef CMI_PROC
+}
+
+static void launch_plan(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
+                        bool track) {
+  TilesEngine *e = c->tiles;
+  hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
+                     G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
+                     e->total, e->ticket, R.chunk, e->d_tile_prob, e->nprob,
+                     track ? e->d_prob_round : nullptr, e->d_prob_time,
+                     e->d_round_ctr);
+}
+
+// Rounds of process / plan / scatter until no packet is left (host-checked
+// after every round).  Round 0's input (records, destinations, histogram)
+// is already in P[0], dest[0], hist; n0 packets.
 static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
                       PacketOut po, uint32_t *rounds_out) {
   TilesEngine *e = c->tiles;
-  const int threads = c->tuning.block_threads == 256 ? 256 : 512;
-  const int refill = c->tuning.refill_batch > 0 ? std::min(64, c->tuning.refill_batch) : 32;
-  const uint32_t chunk = c->tuning.chunk_min > 0
-      ? std::max<uint32_t>(kChunkFloor, (uint32_t)c->tuning.chunk_min) : kChunk;
-  const bool fp32acc = c->tuning.lds_fp32 != 0;
-  const size_t lds = (size_t)G.tile_cells * (sizeof(float) + (fp32acc ? sizeof(float) : sizeof(double)));
-  const int wg_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (lds + 1024)));
-  const int grid = c->num_cus * wg_per_cu;
-  const size_t per_block = (size_t)kScatterThreads * kScatterItems;
-  const unsigned scatter_blocks =
-      (unsigned)std::min<size_t>(8192, (n0 + per_block - 1) / per_block + 1);
-  auto launch_process = [&]() {
-#define CMI_PROC(ACC, TH, LIST)                                                  \
-  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST>), dim3(grid), dim3(TH), \
-                     lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc,   \
-                     e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,   \
-                     LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po, \
-                     c->d_cnt, refill)
-    if (with_pid) {
-      if (fp32acc) {
-        if (threads == 256) CMI_PROC(float, 256, true); else CMI_PROC(float, 512, true);
-      } else {
-        if (threads == 256) CMI_PROC(double, 256, true); else CMI_PROC(double, 512, true);
-      }
-    } else {
-      if (fp32acc) {
-        if (threads == 256) CMI_PROC(float, 256, false); else CMI_PROC(float, 512, false);
-      } else {
-        if (threads == 256) CMI_PROC(double, 256, false); else CMI_PROC(double, 512, false);
-      }
-    }
-#undef CMI_PROC
-  };
-  // plan round 0
-  hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
-                     G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
-                     e->total, e->ticket, chunk);
-  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads),
+  const RoundCfg R = round_cfg(c, G, n0);
+  launch_plan(c, G, R, false);
+  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
                      0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
-  const int max_rounds = 4 * (G.ntl[0] + G.ntl[1] + G.ntl[2]) + 16;
+  const int max_rounds = max_rounds_of(e->probs) + 2;
   int r = 0;
   for (; r < max_rounds; ++r) {
     // round r reads P[0] (through idx) and writes its survivors to P[1];
     // the buffers swap roles before the next round
     CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r], c->stream));
-    launch_process();
+    launch_process(c, G, R, with_pid, po, nullptr);
     CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r + 1], c->stream));
     CMI_TRY_HIP(c, hipGetLastError());
     // total[0] = packets processed this round = slots written to P[1];
     // keep it in total[1] for the scatter, then plan the next round
-    CMI_TRY_HIP(c, hipMemcpyAsync(e->total + 1, e->total, sizeof(uint32_t),
-                                  hipMemcpyDeviceToDevice, c->stream));
-    hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
-                       G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
-                       e->total, e->ticket, chunk);
+    hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, c->stream, e->total);
+    launch_plan(c, G, R, false);
     CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream));
     CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
@@ -876,7 +1171,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
       ++r;
       break;
     }
-    hipLaunchKernelGGL(tiles_scatter_kernel, dim3(scatter_blocks), dim3(kScatterThreads),
+    hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
                        0, c->stream, e->total + 1, e->dest[1], e->seg, e->cursor,
                        e->idx);
     CMI_TRY_HIP(c, hipGetLastError());
@@ -897,6 +1192,36 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
   return CMI_OK;
 }
 
+static void fill_set_sources_single(cmi_ctx *c, TilesEngine *e, uint64_t seed,
+                                    uint32_t iteration) {
+  SetSources &S = *e->h_src;
+  const SourcesDev &H = *c->h_src;
+  S.nprob = 1;
+  S.seed = seed;
+  S.iteration = iteration;
+  S.pk_cum[0] = H.cum[H.nsrc - 1];
+  S.src_begin[0] = 0;
+  S.src_begin[1] = H.nsrc;
+  for (int s = 0; s < H.nsrc; ++s) {
+    S.src_cum[s] = H.cum[s];
+    for (int a = 0; a < 3; ++a) S.pos[s][a] = H.pos[s][a];
+  }
+}
+
+static void fill_stats_tiles(cmi_ctx *c, cmi_stats *stats, uint64_t n,
+                             uint32_t rounds, double seconds) {
+  if (!stats) return;
+  stats->packets = n;
+  stats->crossings = c->h_cnt->crossings;
+  stats->hops = c->h_cnt->hops;
+  stats->absorbed = c->h_cnt->absorbed;
+  stats->escaped = c->h_cnt->escaped;
+  stats->errors = c->h_cnt->errors;
+  stats->rounds = rounds;
+  stats->reserved = 0;
+  stats->seconds = seconds;
+}
+
 int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
                     uint32_t iteration, double, cmi_stats *stats) {
   if (c->precision != CMI_PREC_FP32)
@@ -905,18 +1230,20 @@ int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
     return fail(c, CMI_ERR_ARG, "tiles engine: 1 .. 2^32-16 packets per call");
   for (int a = 0; a < 3; ++a)
     if (c->ncell[a] > 1024) return fail(c, CMI_ERR_ARG, "tiles engine: at most 1024 cells per axis");
-  int rc = ensure_engine(c, (size_t)npackets, false);
+  int rc = ensure_engine(c, ctx_problems(c), (size_t)npackets, false);
   if (rc != CMI_OK) return rc;
   TilesEngine *e = c->tiles;
-  const TileGeo G = make_geo(c);
+  const TileGeo G = make_geo(c, e);
+  fill_set_sources_single(c, e, seed, iteration);
+  CMI_TRY_HIP(c, hipMemcpyAsync(e->d_src, e->h_src, sizeof(SetSources),
+                                hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt);
   CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
   const unsigned eb = (unsigned)((npackets + 255) / 256);
-  hipLaunchKernelGGL(tiles_emit_kernel, dim3(eb), dim3(256), 0, c->stream, G, c->d_src,
-                     npackets, iteration, (uint32_t)seed, (uint32_t)(seed >> 32),
-                     e->P[0], e->dest[0]);
-  hipLaunchKernelGGL(tiles_source_hist_kernel, dim3(1), dim3(kMaxSources), 0,
-                     c->stream, G, c->d_src, e->hist);
+  hipLaunchKernelGGL(tiles_emit_kernel, dim3(eb), dim3(256), 0, c->stream, G, e->d_src,
+                     npackets, e->P[0], e->dest[0]);
+  hipLaunchKernelGGL(tiles_source_hist_kernel, dim3(1), dim3(256), 0, c->stream, G,
+                     e->d_src, e->hist);
   CMI_TRY_HIP(c, hipGetLastError());
   PacketOut po{};
   uint32_t rounds = 0;
@@ -928,17 +1255,7 @@ int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
   CMI_TRY_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-  if (stats) {
-    stats->packets = npackets;
-    stats->crossings = c->h_cnt->crossings;
-    stats->hops = c->h_cnt->hops;
-    stats->absorbed = c->h_cnt->absorbed;
-    stats->escaped = c->h_cnt->escaped;
-    stats->errors = c->h_cnt->errors;
-    stats->rounds = rounds;
-    stats->reserved = 0;
-    stats->seconds = ms * 1e-3;
-  }
+  fill_stats_tiles(c, stats, npackets, rounds, ms * 1e-3);
   if (c->h_cnt->errors)
     return fail(c, CMI_ERR_DIRECTION, "tiles engine: packets stopped on a walk bound");
   return CMI_OK;
@@ -950,10 +1267,10 @@ int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub
   if (c->precision != CMI_PREC_FP32)
     return fail(c, CMI_ERR_ARG, "the tiles engine runs FP32 contexts");
   if (n > 0xfffffff0ull) return fail(c, CMI_ERR_ARG, "too many packets");
-  int rc = ensure_engine(c, n, true);
+  int rc = ensure_engine(c, ctx_problems(c), n, true);
   if (rc != CMI_OK) return rc;
   TilesEngine *e = c->tiles;
-  const TileGeo G = make_geo(c);
+  const TileGeo G = make_geo(c, e);
   PhotonRec *dp = nullptr;
   uint32_t *ds = nullptr;
   int32_t *dst = nullptr, *dtile = nullptr;
@@ -1015,20 +1332,12 @@ int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub
   if (err == hipSuccess) err = hipEventElapsedTime(&ms, c->ev0, c->ev1);
   cleanup();
   if (err != hipSuccess) return fail(c, CMI_ERR_HIP, "tiles list transport failed");
-  if (stats) {
-    stats->packets = n;
-    stats->crossings = c->h_cnt->crossings;
-    stats->hops = c->h_cnt->hops;
-    stats->absorbed = c->h_cnt->absorbed;
-    stats->escaped = c->h_cnt->escaped;
-    stats->errors = c->h_cnt->errors;
-    stats->rounds = rounds;
-    stats->reserved = 0;
-    stats->seconds = ms * 1e-3;
-  }
+  fill_stats_tiles(c, stats, n, rounds, ms * 1e-3);
   if (c->h_cnt->errors)
     return fail(c, CMI_ERR_DIRECTION, "tiles engine: packets stopped on a walk bound");
   return CMI_OK;
 }
 
 }  // namespace cmi
+
+#include "cmi_set.inc"
