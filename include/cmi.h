@@ -241,6 +241,43 @@ int cmi_batch_iteration(cmi_batch *b, const uint64_t *npackets, uint64_t seed,
                         double *l1_change, double *problem_seconds,
                         uint64_t *problem_crossings, cmi_stats *stats);
 
+/* ---- learned warm start ----
+ * A 4-layer 3-D convolutional network (3x3x3 kernels, 32 channels, ReLU;
+ * no counterpart in the reference) from (log10 n_H, log10 of the first-pass
+ * photoionization rate jH, s^-1) to the logit of the converged neutral
+ * fraction, trained and run on BF16 MFMA with FP32 accumulation.  The plain
+ * iteration stays the judge of convergence; the network only supplies the
+ * starting x_n.  Feature arrays are [..][D][H][W][2] (FP32), target and logit
+ * arrays [..][D][H][W]; the last extent W must be a multiple of 32.  Master
+ * parameters are FP32, per layer l: W[27][NOUT][32] (tap = 9 kd + 3 kh + kw,
+ * output channel, input channel; zero beyond the real 2 or 32 input and 32 or
+ * 1 output channels) then b[NOUT], NOUT = 32 (16 for the last layer). */
+typedef struct cmi_net cmi_net;
+int cmi_net_create(uint64_t seed, cmi_net **out);
+void cmi_net_destroy(cmi_net *net);
+const char *cmi_net_last_error(const cmi_net *net);
+size_t cmi_net_param_count(const cmi_net *net);
+int cmi_net_get_params(cmi_net *net, float *params);
+int cmi_net_set_params(cmi_net *net, const float *params);
+/* features are normalised as (f - mean) * scale before the first layer */
+int cmi_net_set_normalisation(cmi_net *net, const float mean[2],
+                              const float scale[2]);
+/* append training samples (kept on the device) */
+int cmi_net_add_samples(cmi_net *net, const float *features,
+                        const float *target_logits, int nsamples,
+                        const int dim[3]);
+int cmi_net_sample_count(const cmi_net *net);
+/* Adam steps on random crops of the samples (mean squared error on logits);
+ * losses (may be NULL) receives one value per step */
+int cmi_net_train(cmi_net *net, int steps, int batch, const int crop[3],
+                  float lr, uint64_t seed, float *losses);
+/* loss and FP32 parameter gradient of one batch, no update */
+int cmi_net_loss_and_grad(cmi_net *net, const float *features,
+                          const float *target_logits, int batch,
+                          const int dim[3], float *loss, float *grad);
+int cmi_net_predict(cmi_net *net, const float *features, const int dim[3],
+                    float *logits);
+
 #ifdef __cplusplus
 }
 #endif

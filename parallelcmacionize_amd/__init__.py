@@ -13,6 +13,7 @@ import numpy as np
 __all__ = [
     "CMIError", "Context", "PHOTON_DTYPE", "load_library", "PREC_FP64",
     "PREC_FP32", "ENGINE_WALK", "ENGINE_TILES", "INSIDE", "output_to_input", "Batch",
+    "WarmStartNet",
     "direction_from_mask", "STATS_FIELDS",
 ]
 
@@ -66,6 +67,7 @@ _u32p = C.POINTER(C.c_uint32)
 _i32p = C.POINTER(C.c_int32)
 _u64p = C.POINTER(C.c_uint64)
 _vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
 
 # entry point -> (restype, argtypes); every symbol include/cmi.h declares
 SIGNATURES = {
@@ -105,6 +107,20 @@ SIGNATURES = {
     "cmi_batch_iteration": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint32,
                                       C.c_double, C.c_int, _dp, _dp, _u64p,
                                       C.POINTER(Stats)]),
+    "cmi_net_create": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
+    "cmi_net_destroy": (None, [_vp]),
+    "cmi_net_last_error": (C.c_char_p, [_vp]),
+    "cmi_net_param_count": (C.c_size_t, [_vp]),
+    "cmi_net_get_params": (C.c_int, [_vp, _fp]),
+    "cmi_net_set_params": (C.c_int, [_vp, _fp]),
+    "cmi_net_set_normalisation": (C.c_int, [_vp, _fp, _fp]),
+    "cmi_net_add_samples": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip]),
+    "cmi_net_sample_count": (C.c_int, [_vp]),
+    "cmi_net_train": (C.c_int, [_vp, C.c_int, C.c_int, _ip, C.c_float,
+                                C.c_uint64, _fp]),
+    "cmi_net_loss_and_grad": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip, _fp,
+                                        _fp]),
+    "cmi_net_predict": (C.c_int, [_vp, _fp, _ip, _fp]),
 }
 
 
@@ -378,6 +394,88 @@ class Batch:
             C.byref(s)))
         return {"l1": l1, "problem_seconds": secs, "problem_crossings": cross,
                 **_stats_dict(s)}
+
+
+class WarmStartNet:
+    """The learned warm start (cmi_net_*): 4 conv layers, 32 channels, BF16
+    MFMA with FP32 accumulation, trained on the GPU."""
+
+    def __init__(self, seed=1):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.cmi_net_create(int(seed), C.byref(h))
+        if rc != 0:
+            raise CMIError(rc, "cmi_net_create failed")
+        self.h = h
+        self.nparam = self.lib.cmi_net_param_count(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cmi_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CMIError(rc, self.lib.cmi_net_last_error(self.h).decode())
+
+    def params(self):
+        out = np.zeros(self.nparam, np.float32)
+        self._check(self.lib.cmi_net_get_params(self.h, _p(out, _fp)))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.nparam
+        self._check(self.lib.cmi_net_set_params(self.h, _p(a, _fp)))
+
+    def set_normalisation(self, mean, scale):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(scale, dtype=np.float32)
+        self._check(self.lib.cmi_net_set_normalisation(self.h, _p(m, _fp),
+                                                       _p(s, _fp)))
+
+    def add_samples(self, features, targets):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        dim = np.array(t.shape[1:], np.int32)
+        self._check(self.lib.cmi_net_add_samples(self.h, _p(f, _fp), _p(t, _fp),
+                                                 t.shape[0], _p(dim, _ip)))
+
+    def sample_count(self):
+        return self.lib.cmi_net_sample_count(self.h)
+
+    def train(self, steps, batch, crop, lr, seed=1):
+        losses = np.zeros(int(steps), np.float32)
+        c = np.ascontiguousarray(crop, dtype=np.int32)
+        self._check(self.lib.cmi_net_train(self.h, int(steps), int(batch),
+                                           _p(c, _ip), float(lr), int(seed),
+                                           _p(losses, _fp)))
+        return losses
+
+    def loss_and_grad(self, features, targets):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        dim = np.array(t.shape[1:], np.int32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        self._check(self.lib.cmi_net_loss_and_grad(
+            self.h, _p(f, _fp), _p(t, _fp), t.shape[0], _p(dim, _ip),
+            _p(loss, _fp), _p(g, _fp)))
+        return float(loss[0]), g
+
+    def predict(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        dim = np.array(f.shape[:3], np.int32)
+        out = np.zeros(f.shape[:3], np.float32)
+        self._check(self.lib.cmi_net_predict(self.h, _p(f, _fp), _p(dim, _ip),
+                                             _p(out, _fp)))
+        return out
 
 
 def _stats_dict(s):

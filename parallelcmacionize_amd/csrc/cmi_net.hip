@@ -1,0 +1,702 @@
+// cmi_net.hip -- the learned warm start: a 4-layer 3-D convolutional network
+// (3x3x3 kernels, 32 channels, ReLU) that maps (log n_H, log of the
+// first-pass photoionization rate) to the logit of the converged neutral
+// fraction.  Trained and run on the GPU by the kernels below:
+//
+//   conv3_kernel   implicit-GEMM 3-D convolution on BF16 MFMA
+//                  (v_mfma_f32_16x16x32_bf16, FP32 accumulation): one wave
+//                  computes 16 voxels (a run along the last axis) x 16 or 32
+//                  output channels; K = 27 taps x 32 input channels, one MFMA
+//                  per tap and output block.  Serves the forward pass
+//                  (weights [tap][cout][cin], bias, ReLU) and the input
+//                  gradient (the same convolution over the output gradient
+//                  with the flipped, transposed weights, masked by the ReLU of
+//                  the layer below).
+//   wgrad_kernel   weight gradient dW[tap][cout][cin] = sum_v dy[v][cout] *
+//                  x[v + tap][cin] on the same MFMA, K = 32 voxels per
+//                  instruction; FP32 accumulation, one atomic add per weight
+//                  per wave.
+//   plus the loss (mean squared error on logits), bias gradients, Adam on
+//   FP32 master weights, weight packing to BF16, crop sampling and input
+//   normalisation.
+//
+// Activations are BF16, channels-last with 32 channels ([B][D][H][W][32]);
+// the first layer's 2 input channels and the last layer's single output are
+// zero-padded to the MFMA shapes.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cmi.h"
+#include "cmi_device.hpp"
+
+namespace cmi {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+constexpr int kC = 32;        // channels of every activation tensor
+constexpr int kTaps = 27;
+constexpr int kLayers = 4;
+// real input / output channels of each layer
+__host__ __device__ constexpr int layer_cin(int l) { return l == 0 ? 2 : 32; }
+__host__ __device__ constexpr int layer_cout(int l) { return l == kLayers - 1 ? 1 : 32; }
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+__device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
+
+// ---------------------------------------------------------------------------
+// 3-D convolution (padding 1) as implicit GEMM on MFMA 16x16x32 BF16.
+// Lane l of a wave: A rows (voxels) l%16, A/B k-group (input channels)
+// 8*(l/16)..+7, B column (output channel) l%16; accumulator rows (voxels)
+// 4*(l/16)..+3, column l%16.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ void __launch_bounds__(256) conv3_kernel(
+    const __bf16 *__restrict__ x, const __bf16 *__restrict__ w,
+    const float *__restrict__ bias, int nrun, int D, int H, int W,
+    __bf16 *__restrict__ y, float *__restrict__ yf,
+    const __bf16 *__restrict__ mask, int relu) {
+  const int lane = threadIdx.x & 63;
+  const int run = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (run >= nrun) return;  // whole wave leaves together
+  // run -> (b, d, h, w0): 16 voxels along the last axis
+  const int wr = W >> 4;
+  int t = run;
+  const int w0 = (t % wr) << 4;
+  t /= wr;
+  const int h = t % H;
+  t /= H;
+  const int d = t % D;
+  const int b = t / D;
+  const int vl = lane & 15;   // voxel (A row) / output channel (B column)
+  const int kg = lane >> 4;   // k group: channels 8*kg .. 8*kg+7
+  f32x4 acc[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int NOUT = 16 * NB;
+#pragma unroll 1
+  for (int tap = 0; tap < kTaps; ++tap) {
+    const int kd = tap / 9, kh = (tap / 3) % 3, kw = tap % 3;
+    const int zd = d + kd - 1, zh = h + kh - 1, zw = w0 + vl + kw - 1;
+    bf16x8 a;
+    if ((unsigned)zd < (unsigned)D && (unsigned)zh < (unsigned)H &&
+        (unsigned)zw < (unsigned)W) {
+      const size_t vox = (((size_t)b * D + zd) * H + zh) * W + zw;
+      a = *reinterpret_cast<const bf16x8 *>(x + vox * kC + 8 * kg);
+    } else {
+      a = bf16x8{};
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const bf16x8 bw = *reinterpret_cast<const bf16x8 *>(
+          w + ((size_t)tap * NOUT + nb * 16 + vl) * kC + 8 * kg);
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc[nb], 0, 0, 0);
+    }
+  }
+  const size_t vbase = (((size_t)b * D + d) * H + h) * W + w0;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) {
+    const int co = nb * 16 + vl;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const size_t v = vbase + 4 * kg + i;
+      float r = acc[nb][i] + (bias ? bias[co] : 0.f);
+      if (mask) r = bf2f(mask[v * kC + co]) > 0.f ? r : 0.f;
+      if (relu) r = fmaxf(r, 0.f);
+      if (y) y[v * kC + co] = f2bf(r);
+      if (yf && co == 0) yf[v] = r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[tap][cout][cin] += sum over voxels of
+// dy[v][cout] * x[v + tap - 1][cin].  A wave owns (tap, cout block) and both
+// cin blocks and sweeps `rows_per_wave` runs of 32 voxels.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) wgrad_kernel(
+    const __bf16 *__restrict__ dy, const __bf16 *__restrict__ x, int nb_out,
+    int nrow, int rows_per_wave, int D, int H, int W, float *__restrict__ dW,
+    int NOUT) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nchunks = (nrow + rows_per_wave - 1) / rows_per_wave;
+  const int job = wave;
+  if (job >= kTaps * nb_out * nchunks) return;
+  const int chunk = job % nchunks;
+  const int mb = (job / nchunks) % nb_out;
+  const int tap = job / (nchunks * nb_out);
+  const int kd = tap / 9, kh = (tap / 3) % 3, kw = tap % 3;
+  const int ml = lane & 15;   // A row: cout; B column: cin
+  const int kg = lane >> 4;   // voxels 8*kg .. 8*kg+7 of the 32-voxel run
+  f32x4 acc0 = f32x4{0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+  const int wr = W >> 5;
+  const int r0 = chunk * rows_per_wave, r1 = min(nrow, r0 + rows_per_wave);
+  for (int row = r0; row < r1; ++row) {
+    int t = row;
+    const int w0 = (t % wr) << 5;
+    t /= wr;
+    const int h = t % H;
+    t /= H;
+    const int d = t % D;
+    const int b = t / D;
+    const size_t vbase = (((size_t)b * D + d) * H + h) * W + w0 + 8 * kg;
+    bf16x8 a, b0, b1;
+    const int zd = d + kd - 1, zh = h + kh - 1;
+    const bool rowin = (unsigned)zd < (unsigned)D && (unsigned)zh < (unsigned)H;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      a[j] = dy[(vbase + j) * kC + mb * 16 + ml];
+      const int zw = w0 + 8 * kg + j + kw - 1;
+      if (rowin && (unsigned)zw < (unsigned)W) {
+        const size_t vx = (((size_t)b * D + zd) * H + zh) * W + zw;
+        b0[j] = x[vx * kC + ml];
+        b1[j] = x[vx * kC + 16 + ml];
+      } else {
+        b0[j] = (__bf16)0.f;
+        b1[j] = (__bf16)0.f;
+      }
+    }
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
+  }
+  // accumulator: rows (cout) 4*kg..+3, column (cin) ml
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int co = mb * 16 + 4 * kg + i;
+    atomicAdd(&dW[((size_t)tap * NOUT + co) * kC + ml], acc0[i]);
+    atomicAdd(&dW[((size_t)tap * NOUT + co) * kC + 16 + ml], acc1[i]);
+  }
+}
+
+// db[c] += sum over voxels of dy[v][c], c < nc
+__global__ void bgrad_kernel(const __bf16 *__restrict__ dy, size_t nvox, int nc,
+                             float *__restrict__ db) {
+  __shared__ float sh[kC];
+  if (threadIdx.x < kC) sh[threadIdx.x] = 0.f;
+  __syncthreads();
+  const int c = threadIdx.x & 31;
+  float s = 0.f;
+  if (c < nc) {
+    for (size_t v = (size_t)blockIdx.x * (blockDim.x >> 5) + (threadIdx.x >> 5); v < nvox;
+         v += (size_t)gridDim.x * (blockDim.x >> 5))
+      s += bf2f(dy[v * kC + c]);
+    atomicAdd(&sh[c], s);
+  }
+  __syncthreads();
+  if (threadIdx.x < nc) atomicAdd(&db[threadIdx.x], sh[threadIdx.x]);
+}
+
+// Mean squared error on logits: loss += sum (z - t)^2 / n; dy[v][0] =
+// 2 (z - t) / n, other channels 0.
+__global__ void mse_kernel(const float *__restrict__ z, const float *__restrict__ tgt,
+                           size_t n, __bf16 *__restrict__ dy, float *__restrict__ loss) {
+  __shared__ float sh[4];
+  float s = 0.f;
+  const float inv = 1.f / (float)n;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const float e = z[v] - tgt[v];
+    s += e * e;
+    __bf16 *o = dy + v * kC;
+    o[0] = f2bf(2.f * e * inv);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+    atomicAdd(loss, t * inv);
+  }
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master parameters.
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g,
+                            float *__restrict__ m, float *__restrict__ v, size_t n,
+                            float lr, float b1, float b2, float eps, float c1,
+                            float c2, float wd) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float gi = g[i] + wd * p[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  }
+}
+
+// FP32 master layout of layer l (padded to the MFMA shapes): W[27][NOUT][32]
+// (tap, output channel, input channel; entries beyond the real channels stay
+// 0: their gradients are 0) then b[NOUT].  Packed BF16 GEMM weights: fwd =
+// W as is, and the input-gradient weights bwd[t'][cin][cout] = W[26-t'][cout]
+// [cin] ([27][32][32], zero for cout >= NOUT).
+__global__ void pack_kernel(const float *__restrict__ master, int NOUT,
+                            __bf16 *__restrict__ fwd, __bf16 *__restrict__ bwd) {
+  const int n = kTaps * 32 * 32;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int tap = i / 1024, o = (i / 32) % 32, k = i % 32;
+    if (o < NOUT) fwd[((size_t)tap * NOUT + o) * kC + k] = f2bf(master[((size_t)tap * NOUT + o) * kC + k]);
+    const float vb = k < NOUT ? master[((size_t)(kTaps - 1 - tap) * NOUT + k) * kC + o] : 0.f;
+    bwd[((size_t)tap * 32 + o) * kC + k] = f2bf(vb);
+  }
+}
+
+// Input features of one sample (log10 n_H, log10 jH, FP32) -> BF16 channels
+// 0, 1 of the activation tensor (normalised), zeros elsewhere; an optional
+// crop [o..o+S) of a larger sample.
+__global__ void features_kernel(const float *__restrict__ feat, int D, int H, int W,
+                                int od, int oh, int ow, int S0, int S1, int S2,
+                                float m0, float s0, float m1, float s1,
+                                __bf16 *__restrict__ out) {
+  const size_t n = (size_t)S0 * S1 * S2;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * kC;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t v = i / kC;
+    const int c = (int)(i % kC);
+    float r = 0.f;
+    if (c < 2) {
+      const int z = (int)(v % S2), y = (int)((v / S2) % S1), xx = (int)(v / ((size_t)S1 * S2));
+      const size_t src = (((size_t)(od + xx) * H) + (oh + y)) * W + (ow + z);
+      r = c == 0 ? (feat[2 * src] - m0) * s0 : (feat[2 * src + 1] - m1) * s1;
+    }
+    out[i] = f2bf(r);
+  }
+}
+
+__global__ void crop_target_kernel(const float *__restrict__ t, int D, int H, int W,
+                                   int od, int oh, int ow, int S0, int S1, int S2,
+                                   float *__restrict__ out) {
+  const size_t n = (size_t)S0 * S1 * S2;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(v % S2), y = (int)((v / S2) % S1), xx = (int)(v / ((size_t)S1 * S2));
+    out[v] = t[(((size_t)(od + xx) * H) + (oh + y)) * W + (ow + z)];
+  }
+}
+
+__global__ void zero_f32_kernel(float *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
+
+__global__ void zero_bf16_kernel(__bf16 *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = f2bf(0.f);
+}
+
+}  // namespace cmi
+
+using namespace cmi;
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct cmi_net {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  size_t nparam = 0;
+  size_t off_w[kLayers], off_b[kLayers];  // offsets into the master vector
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // FP32
+  __bf16 *wf[kLayers] = {}, *wb[kLayers] = {};                    // packed
+  int step = 0;
+  // normalisation of the features and of the target logits
+  float mean[2] = {8.f, -10.f}, scale[2] = {1.f, 0.25f};
+  // dataset (device): features [S][N][2] FP32, targets [S][N] FP32
+  float *ds_feat = nullptr, *ds_tgt = nullptr;
+  int ds_n = 0, ds_dim[3] = {0, 0, 0};
+  size_t ds_cap = 0;
+  // work buffers (sized for the largest batch seen)
+  size_t work_vox = 0;
+  __bf16 *act[kLayers] = {};  // act[0] = input, act[l] = output of layer l-1
+  __bf16 *dact[2] = {};       // gradient ping-pong
+  float *z = nullptr, *tgt = nullptr, *loss = nullptr;
+  float *feat_tmp = nullptr;
+  size_t feat_cap = 0;
+};
+
+namespace {
+
+int nfail(cmi_net *n, int code, const std::string &m) {
+  if (n) n->err = m;
+  return code;
+}
+
+#define NET_TRY(n, call)                                                   \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return nfail(n, CMI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int nout_of(int l) { return layer_cout(l) > 16 ? 32 : 16; }
+
+int ensure_work(cmi_net *n, size_t nvox) {
+  if (nvox <= n->work_vox) return CMI_OK;
+  for (int l = 0; l < kLayers; ++l) hipFree(n->act[l]);
+  for (int k = 0; k < 2; ++k) hipFree(n->dact[k]);
+  hipFree(n->z);
+  hipFree(n->tgt);
+  n->work_vox = 0;
+  for (int l = 0; l < kLayers; ++l) NET_TRY(n, hipMalloc(&n->act[l], nvox * kC * sizeof(__bf16)));
+  for (int k = 0; k < 2; ++k) NET_TRY(n, hipMalloc(&n->dact[k], nvox * kC * sizeof(__bf16)));
+  NET_TRY(n, hipMalloc(&n->z, nvox * sizeof(float)));
+  NET_TRY(n, hipMalloc(&n->tgt, nvox * sizeof(float)));
+  n->work_vox = nvox;
+  return CMI_OK;
+}
+
+int pack(cmi_net *n) {
+  for (int l = 0; l < kLayers; ++l)
+    hipLaunchKernelGGL(pack_kernel, dim3(108), dim3(256), 0, n->stream, n->p + n->off_w[l],
+                       nout_of(l), n->wf[l], n->wb[l]);
+  NET_TRY(n, hipGetLastError());
+  return CMI_OK;
+}
+
+unsigned grid_for(size_t items, unsigned per_block, unsigned cap = 65535) {
+  size_t g = (items + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  return (unsigned)(g < cap ? g : cap);
+}
+
+// forward over act[0] (nvox voxels in B samples of D x H x W) -> z
+int forward(cmi_net *n, int B, int D, int H, int W, float *zout) {
+  const int nrun = B * D * H * (W / 16);
+  const unsigned blocks = grid_for((size_t)nrun, 4);
+  for (int l = 0; l < kLayers; ++l) {
+    const float *bias = n->p + n->off_b[l];
+    if (l < kLayers - 1) {
+      hipLaunchKernelGGL(conv3_kernel<2>, dim3(blocks), dim3(256), 0, n->stream, n->act[l],
+                         n->wf[l], bias, nrun, D, H, W, n->act[l + 1], (float *)nullptr,
+                         (const __bf16 *)nullptr, 1);
+    } else {
+      hipLaunchKernelGGL(conv3_kernel<1>, dim3(blocks), dim3(256), 0, n->stream, n->act[l],
+                         n->wf[l], bias, nrun, D, H, W, (__bf16 *)nullptr, zout,
+                         (const __bf16 *)nullptr, 0);
+    }
+  }
+  NET_TRY(n, hipGetLastError());
+  return CMI_OK;
+}
+
+// backward from the MSE loss on z vs tgt; gradients accumulate into n->g
+int backward(cmi_net *n, int B, int D, int H, int W) {
+  const size_t nvox = (size_t)B * D * H * W;
+  const int nrun = B * D * H * (W / 16);
+  const unsigned blocks = grid_for((size_t)nrun, 4);
+  const int nrow = B * D * H * (W / 32);
+  const int rows_per_wave = 16;
+  const int nchunks = (nrow + rows_per_wave - 1) / rows_per_wave;
+  __bf16 *dy = n->dact[0], *dx = n->dact[1];
+  hipLaunchKernelGGL(zero_bf16_kernel, dim3(grid_for(nvox * kC, 256, 4096)), dim3(256), 0,
+                     n->stream, dy, nvox * kC);
+  hipLaunchKernelGGL(mse_kernel, dim3(grid_for(nvox, 256, 1024)), dim3(256), 0, n->stream,
+                     n->z, n->tgt, nvox, dy, n->loss);
+  for (int l = kLayers - 1; l >= 0; --l) {
+    const int NOUT = nout_of(l);
+    const int nb_out = NOUT / 16;
+    const size_t waves = (size_t)kTaps * nb_out * nchunks;
+    hipLaunchKernelGGL(wgrad_kernel, dim3(grid_for(waves, 4)), dim3(256), 0, n->stream, dy,
+                       n->act[l], nb_out, nrow, rows_per_wave, D, H, W, n->g + n->off_w[l], NOUT);
+    hipLaunchKernelGGL(bgrad_kernel, dim3(grid_for(nvox, 8, 2048)), dim3(256), 0, n->stream,
+                       dy, nvox, layer_cout(l), n->g + n->off_b[l]);
+    if (l > 0) {
+      // dx = conv(dy, flipped transposed W), masked by the ReLU of layer l-1
+      hipLaunchKernelGGL(conv3_kernel<2>, dim3(blocks), dim3(256), 0, n->stream, dy, n->wb[l],
+                         (const float *)nullptr, nrun, D, H, W, dx, (float *)nullptr,
+                         (const __bf16 *)n->act[l], 0);
+      std::swap(dy, dx);
+    }
+  }
+  NET_TRY(n, hipGetLastError());
+  return CMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cmi_net_create(uint64_t seed, cmi_net **out) {
+  if (!out) return CMI_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return CMI_ERR_NODEVICE;
+  cmi_net *n = new cmi_net();
+  (void)hipGetDevice(&n->device);
+  size_t off = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    n->off_w[l] = off;
+    off += (size_t)kTaps * nout_of(l) * kC;
+    n->off_b[l] = off;
+    off += nout_of(l);
+  }
+  n->nparam = off;
+  hipError_t e = hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&n->p, off * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&n->g, off * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&n->m, off * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&n->v, off * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&n->loss, 65536 * sizeof(float));
+  for (int l = 0; l < kLayers && e == hipSuccess; ++l) {
+    e = hipMalloc(&n->wf[l], (size_t)kTaps * nout_of(l) * kC * sizeof(__bf16));
+    if (e == hipSuccess) e = hipMalloc(&n->wb[l], (size_t)kTaps * 32 * kC * sizeof(__bf16));
+  }
+  if (e != hipSuccess) {
+    cmi_net_destroy(n);
+    return CMI_ERR_ALLOC;
+  }
+  // He-uniform initial weights over the real fan-in, zero biases, from a
+  // Philox stream
+  std::vector<float> h(off, 0.f);
+  uint64_t ctr = 0;
+  for (int l = 0; l < kLayers; ++l) {
+    const float bound = sqrtf(6.f / (kTaps * layer_cin(l))) * (l == kLayers - 1 ? 0.5f : 1.f);
+    const int NOUT = nout_of(l);
+    for (int tap = 0; tap < kTaps; ++tap)
+      for (int o = 0; o < layer_cout(l); ++o)
+        for (int k = 0; k < layer_cin(l); ++k) {
+          const u32x4 r = philox4x32_10(u32x4{(uint32_t)ctr, (uint32_t)(ctr >> 32), 0xC0u, 0u},
+                                        (uint32_t)seed, (uint32_t)(seed >> 32));
+          ++ctr;
+          const double u = uniform_open(r.x, r.y);
+          h[n->off_w[l] + ((size_t)tap * NOUT + o) * kC + k] = (float)((2. * u - 1.) * bound);
+        }
+  }
+  e = hipMemcpy(n->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(n->m, 0, off * sizeof(float));
+  if (e == hipSuccess) e = hipMemset(n->v, 0, off * sizeof(float));
+  if (e != hipSuccess || pack(n) != CMI_OK || hipStreamSynchronize(n->stream) != hipSuccess) {
+    cmi_net_destroy(n);
+    return CMI_ERR_HIP;
+  }
+  *out = n;
+  return CMI_OK;
+}
+
+void cmi_net_destroy(cmi_net *n) {
+  if (!n) return;
+  (void)hipSetDevice(n->device);
+  if (n->stream) (void)hipStreamSynchronize(n->stream);
+  hipFree(n->p);
+  hipFree(n->g);
+  hipFree(n->m);
+  hipFree(n->v);
+  hipFree(n->loss);
+  for (int l = 0; l < kLayers; ++l) {
+    hipFree(n->wf[l]);
+    hipFree(n->wb[l]);
+    hipFree(n->act[l]);
+  }
+  for (int k = 0; k < 2; ++k) hipFree(n->dact[k]);
+  hipFree(n->z);
+  hipFree(n->tgt);
+  hipFree(n->ds_feat);
+  hipFree(n->ds_tgt);
+  hipFree(n->feat_tmp);
+  if (n->stream) (void)hipStreamDestroy(n->stream);
+  delete n;
+}
+
+const char *cmi_net_last_error(const cmi_net *n) { return n ? n->err.c_str() : "null net"; }
+
+size_t cmi_net_param_count(const cmi_net *n) { return n ? n->nparam : 0; }
+
+int cmi_net_get_params(cmi_net *n, float *out) {
+  if (!n || !out) return CMI_ERR_ARG;
+  (void)hipSetDevice(n->device);
+  NET_TRY(n, hipStreamSynchronize(n->stream));
+  NET_TRY(n, hipMemcpy(out, n->p, n->nparam * sizeof(float), hipMemcpyDeviceToHost));
+  return CMI_OK;
+}
+
+int cmi_net_set_params(cmi_net *n, const float *in) {
+  if (!n || !in) return CMI_ERR_ARG;
+  (void)hipSetDevice(n->device);
+  NET_TRY(n, hipMemcpy(n->p, in, n->nparam * sizeof(float), hipMemcpyHostToDevice));
+  NET_TRY(n, hipMemset(n->m, 0, n->nparam * sizeof(float)));
+  NET_TRY(n, hipMemset(n->v, 0, n->nparam * sizeof(float)));
+  n->step = 0;
+  const int rc = pack(n);
+  if (rc != CMI_OK) return rc;
+  NET_TRY(n, hipStreamSynchronize(n->stream));
+  return CMI_OK;
+}
+
+int cmi_net_set_normalisation(cmi_net *n, const float mean[2], const float scale[2]) {
+  if (!n || !mean || !scale) return CMI_ERR_ARG;
+  for (int c = 0; c < 2; ++c) {
+    n->mean[c] = mean[c];
+    n->scale[c] = scale[c];
+  }
+  return CMI_OK;
+}
+
+int cmi_net_add_samples(cmi_net *n, const float *feat, const float *target, int nsamples,
+                        const int dim[3]) {
+  if (!n || !feat || !target || nsamples <= 0 || !dim) return CMI_ERR_ARG;
+  (void)hipSetDevice(n->device);
+  if (n->ds_n > 0 && (dim[0] != n->ds_dim[0] || dim[1] != n->ds_dim[1] || dim[2] != n->ds_dim[2]))
+    return nfail(n, CMI_ERR_ARG, "all samples must have the same shape");
+  const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
+  const size_t need = (size_t)(n->ds_n + nsamples);
+  if (need > n->ds_cap) {
+    size_t cap = std::max(need, 2 * n->ds_cap);
+    float *f2 = nullptr, *t2 = nullptr;
+    NET_TRY(n, hipMalloc(&f2, cap * nv * 2 * sizeof(float)));
+    NET_TRY(n, hipMalloc(&t2, cap * nv * sizeof(float)));
+    if (n->ds_n) {
+      NET_TRY(n, hipMemcpy(f2, n->ds_feat, (size_t)n->ds_n * nv * 2 * sizeof(float),
+                           hipMemcpyDeviceToDevice));
+      NET_TRY(n, hipMemcpy(t2, n->ds_tgt, (size_t)n->ds_n * nv * sizeof(float),
+                           hipMemcpyDeviceToDevice));
+    }
+    hipFree(n->ds_feat);
+    hipFree(n->ds_tgt);
+    n->ds_feat = f2;
+    n->ds_tgt = t2;
+    n->ds_cap = cap;
+  }
+  NET_TRY(n, hipMemcpy(n->ds_feat + (size_t)n->ds_n * nv * 2, feat,
+                       (size_t)nsamples * nv * 2 * sizeof(float), hipMemcpyHostToDevice));
+  NET_TRY(n, hipMemcpy(n->ds_tgt + (size_t)n->ds_n * nv, target,
+                       (size_t)nsamples * nv * sizeof(float), hipMemcpyHostToDevice));
+  for (int a = 0; a < 3; ++a) n->ds_dim[a] = dim[a];
+  n->ds_n += nsamples;
+  return CMI_OK;
+}
+
+int cmi_net_sample_count(const cmi_net *n) { return n ? n->ds_n : 0; }
+
+int cmi_net_train(cmi_net *n, int steps, int batch, const int crop[3], float lr,
+                  uint64_t seed, float *losses) {
+  if (!n || steps <= 0 || steps > 65536 || batch <= 0 || !crop || !(lr > 0.f))
+    return nfail(n, CMI_ERR_ARG, "bad training arguments");
+  if (n->ds_n == 0) return nfail(n, CMI_ERR_STATE, "no samples");
+  for (int a = 0; a < 3; ++a)
+    if (crop[a] <= 0 || crop[a] > n->ds_dim[a]) return nfail(n, CMI_ERR_ARG, "crop larger than samples");
+  if (crop[2] % 32 != 0) return nfail(n, CMI_ERR_ARG, "crop: last extent must be a multiple of 32");
+  (void)hipSetDevice(n->device);
+  const size_t cv = (size_t)crop[0] * crop[1] * crop[2];
+  int rc = ensure_work(n, cv * batch);
+  if (rc != CMI_OK) return rc;
+  const size_t nv = (size_t)n->ds_dim[0] * n->ds_dim[1] * n->ds_dim[2];
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(64), dim3(256), 0, n->stream, n->loss, (size_t)steps);
+  const float b1 = 0.9f, b2 = 0.999f;
+  for (int s = 0; s < steps; ++s) {
+    for (int b = 0; b < batch; ++b) {
+      const u32x4 r = philox4x32_10(u32x4{(uint32_t)s, (uint32_t)b, (uint32_t)n->step, 0x7A1Eu},
+                                    (uint32_t)seed, (uint32_t)(seed >> 32));
+      const int si = (int)(r.x % (uint32_t)n->ds_n);
+      const int od = (int)(r.y % (uint32_t)(n->ds_dim[0] - crop[0] + 1));
+      const int oh = (int)(r.z % (uint32_t)(n->ds_dim[1] - crop[1] + 1));
+      const int ow = (int)(r.w % (uint32_t)(n->ds_dim[2] - crop[2] + 1));
+      hipLaunchKernelGGL(features_kernel, dim3(grid_for(cv * kC, 256, 4096)), dim3(256), 0,
+                         n->stream, n->ds_feat + (size_t)si * nv * 2, n->ds_dim[0],
+                         n->ds_dim[1], n->ds_dim[2], od, oh, ow, crop[0], crop[1], crop[2],
+                         n->mean[0], n->scale[0], n->mean[1], n->scale[1],
+                         n->act[0] + (size_t)b * cv * kC);
+      hipLaunchKernelGGL(crop_target_kernel, dim3(grid_for(cv, 256, 4096)), dim3(256), 0,
+                         n->stream, n->ds_tgt + (size_t)si * nv, n->ds_dim[0], n->ds_dim[1],
+                         n->ds_dim[2], od, oh, ow, crop[0], crop[1], crop[2],
+                         n->tgt + (size_t)b * cv);
+    }
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(64), dim3(256), 0, n->stream, n->g, n->nparam);
+    rc = forward(n, batch, crop[0], crop[1], crop[2], n->z);
+    if (rc != CMI_OK) return rc;
+    float *loss_slot = n->loss + s;
+    std::swap(n->loss, loss_slot);  // backward accumulates into n->loss
+    rc = backward(n, batch, crop[0], crop[1], crop[2]);
+    std::swap(n->loss, loss_slot);
+    if (rc != CMI_OK) return rc;
+    ++n->step;
+    const float c1 = 1.f - powf(b1, (float)n->step), c2 = 1.f - powf(b2, (float)n->step);
+    hipLaunchKernelGGL(adam_kernel, dim3(128), dim3(256), 0, n->stream, n->p, n->g, n->m, n->v,
+                       n->nparam, lr, b1, b2, 1e-8f, c1, c2, 0.f);
+    rc = pack(n);
+    if (rc != CMI_OK) return rc;
+  }
+  NET_TRY(n, hipGetLastError());
+  if (losses)
+    NET_TRY(n, hipMemcpyAsync(losses, n->loss, steps * sizeof(float), hipMemcpyDeviceToHost,
+                              n->stream));
+  NET_TRY(n, hipStreamSynchronize(n->stream));
+  return CMI_OK;
+}
+
+// One forward + backward on the given batch (features [B][D][H][W][2]
+// log10 n_H, log10 jH; targets [B][D][H][W] logits): the loss and the FP32
+// gradient of every master parameter, no update.
+int cmi_net_loss_and_grad(cmi_net *n, const float *feat, const float *target, int B,
+                          const int dim[3], float *loss, float *grad) {
+  if (!n || !feat || !target || B <= 0 || !dim) return CMI_ERR_ARG;
+  if (dim[2] % 32 != 0) return nfail(n, CMI_ERR_ARG, "last extent must be a multiple of 32");
+  (void)hipSetDevice(n->device);
+  const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
+  int rc = ensure_work(n, nv * B);
+  if (rc != CMI_OK) return rc;
+  float *df = nullptr;
+  NET_TRY(n, hipMalloc(&df, nv * B * 2 * sizeof(float)));
+  hipError_t e = hipMemcpy(df, feat, nv * B * 2 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(n->tgt, target, nv * B * sizeof(float), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(df);
+    return nfail(n, CMI_ERR_HIP, "loss_and_grad: copy failed");
+  }
+  for (int b = 0; b < B; ++b)
+    hipLaunchKernelGGL(features_kernel, dim3(grid_for(nv * kC, 256, 4096)), dim3(256), 0,
+                       n->stream, df + (size_t)b * nv * 2, dim[0], dim[1], dim[2], 0, 0, 0,
+                       dim[0], dim[1], dim[2], n->mean[0], n->scale[0], n->mean[1],
+                       n->scale[1], n->act[0] + (size_t)b * nv * kC);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(64), dim3(256), 0, n->stream, n->g, n->nparam);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, n->stream, n->loss, (size_t)1);
+  rc = forward(n, B, dim[0], dim[1], dim[2], n->z);
+  if (rc == CMI_OK) rc = backward(n, B, dim[0], dim[1], dim[2]);
+  if (rc == CMI_OK) {
+    e = hipMemcpyAsync(grad, n->g, n->nparam * sizeof(float), hipMemcpyDeviceToHost, n->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+    if (e != hipSuccess) rc = nfail(n, CMI_ERR_HIP, "loss_and_grad failed");
+  }
+  hipFree(df);
+  return rc;
+}
+
+// Logits of x_n for one grid (features [D][H][W][2]).
+int cmi_net_predict(cmi_net *n, const float *feat, const int dim[3], float *logits) {
+  if (!n || !feat || !dim || !logits) return CMI_ERR_ARG;
+  if (dim[2] % 32 != 0) return nfail(n, CMI_ERR_ARG, "last extent must be a multiple of 32");
+  (void)hipSetDevice(n->device);
+  const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
+  int rc = ensure_work(n, nv);
+  if (rc != CMI_OK) return rc;
+  if (n->feat_cap < nv) {
+    hipFree(n->feat_tmp);
+    n->feat_tmp = nullptr;
+    n->feat_cap = 0;
+    NET_TRY(n, hipMalloc(&n->feat_tmp, nv * 2 * sizeof(float)));
+    n->feat_cap = nv;
+  }
+  NET_TRY(n, hipMemcpy(n->feat_tmp, feat, nv * 2 * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(features_kernel, dim3(grid_for(nv * kC, 256, 8192)), dim3(256), 0,
+                     n->stream, n->feat_tmp, dim[0], dim[1], dim[2], 0, 0, 0, dim[0], dim[1],
+                     dim[2], n->mean[0], n->scale[0], n->mean[1], n->scale[1], n->act[0]);
+  rc = forward(n, 1, dim[0], dim[1], dim[2], n->z);
+  if (rc != CMI_OK) return rc;
+  NET_TRY(n, hipMemcpyAsync(logits, n->z, nv * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+  NET_TRY(n, hipStreamSynchronize(n->stream));
+  return CMI_OK;
+}
+
+}  // extern "C"
