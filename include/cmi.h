@@ -259,9 +259,11 @@ const char *cmi_net_last_error(const cmi_net *net);
 size_t cmi_net_param_count(const cmi_net *net);
 int cmi_net_get_params(cmi_net *net, float *params);
 int cmi_net_set_params(cmi_net *net, const float *params);
-/* features are normalised as (f - mean) * scale before the first layer */
+/* features are normalised as (f - mean) * scale before the first layer; the
+ * last layer's output is the logit divided by logit_scale (training targets
+ * are divided by it, predictions multiplied) */
 int cmi_net_set_normalisation(cmi_net *net, const float mean[2],
-                              const float scale[2]);
+                              const float scale[2], float logit_scale);
 /* append training samples (kept on the device) */
 int cmi_net_add_samples(cmi_net *net, const float *features,
                         const float *target_logits, int nsamples,
@@ -277,6 +279,19 @@ int cmi_net_loss_and_grad(cmi_net *net, const float *features,
                           const int dim[3], float *loss, float *grad);
 int cmi_net_predict(cmi_net *net, const float *features, const int dim[3],
                     float *logits);
+/* Warm start of a context, all on the device.  The context's J must hold the
+ * deposits of a first transport of `npackets` packets at total luminosity
+ * `luminosity`; per cell jH = L / N / V_cell * J (the jfac of
+ * cmi_update_ionization).  Features: log10 n_H and log10(jH + 1e-12 s^-1).
+ * The network's logits set x_n = 1 / (1 + exp(-logit)), clamped to
+ * [1e-7, 1]; J is zeroed and the FP32 opacity refreshed.  ncell[2] of the
+ * context must be a multiple of 16. */
+int cmi_net_warm_start(cmi_net *net, cmi_ctx *ctx, double luminosity,
+                       uint64_t npackets);
+/* The same features of a context's current J, as [D][H][W][2] FP32 on the
+ * host (for building training sets from contexts). */
+int cmi_net_context_features(cmi_net *net, cmi_ctx *ctx, double luminosity,
+                             uint64_t npackets, float *features);
 
 #ifdef __cplusplus
 }

@@ -113,7 +113,7 @@ SIGNATURES = {
     "cmi_net_param_count": (C.c_size_t, [_vp]),
     "cmi_net_get_params": (C.c_int, [_vp, _fp]),
     "cmi_net_set_params": (C.c_int, [_vp, _fp]),
-    "cmi_net_set_normalisation": (C.c_int, [_vp, _fp, _fp]),
+    "cmi_net_set_normalisation": (C.c_int, [_vp, _fp, _fp, C.c_float]),
     "cmi_net_add_samples": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip]),
     "cmi_net_sample_count": (C.c_int, [_vp]),
     "cmi_net_train": (C.c_int, [_vp, C.c_int, C.c_int, _ip, C.c_float,
@@ -121,6 +121,9 @@ SIGNATURES = {
     "cmi_net_loss_and_grad": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip, _fp,
                                         _fp]),
     "cmi_net_predict": (C.c_int, [_vp, _fp, _ip, _fp]),
+    "cmi_net_warm_start": (C.c_int, [_vp, _vp, C.c_double, C.c_uint64]),
+    "cmi_net_context_features": (C.c_int, [_vp, _vp, C.c_double, C.c_uint64,
+                                           _fp]),
 }
 
 
@@ -434,11 +437,11 @@ class WarmStartNet:
         assert a.size == self.nparam
         self._check(self.lib.cmi_net_set_params(self.h, _p(a, _fp)))
 
-    def set_normalisation(self, mean, scale):
+    def set_normalisation(self, mean, scale, logit_scale=8.0):
         m = np.ascontiguousarray(mean, dtype=np.float32)
         s = np.ascontiguousarray(scale, dtype=np.float32)
-        self._check(self.lib.cmi_net_set_normalisation(self.h, _p(m, _fp),
-                                                       _p(s, _fp)))
+        self._check(self.lib.cmi_net_set_normalisation(
+            self.h, _p(m, _fp), _p(s, _fp), float(logit_scale)))
 
     def add_samples(self, features, targets):
         f = np.ascontiguousarray(features, dtype=np.float32)
@@ -475,6 +478,21 @@ class WarmStartNet:
         out = np.zeros(f.shape[:3], np.float32)
         self._check(self.lib.cmi_net_predict(self.h, _p(f, _fp), _p(dim, _ip),
                                              _p(out, _fp)))
+        return out
+
+    def warm_start(self, ctx, luminosity, npackets):
+        """Set ctx's x_n from the network, given ctx's J from a first
+        transport of npackets packets (cmi_net_warm_start)."""
+        self._check(self.lib.cmi_net_warm_start(self.h, ctx.h, float(luminosity),
+                                                int(npackets)))
+
+    def context_features(self, ctx, luminosity, npackets):
+        """(log10 n_H, log10(jH + 1e-12)) of ctx's current J as a
+        (nx, ny, nz, 2) float32 lattice (cmi_net_context_features)."""
+        shape = tuple(int(v) for v in ctx.ncell) + (2,)
+        out = np.zeros(shape, np.float32)
+        self._check(self.lib.cmi_net_context_features(
+            self.h, ctx.h, float(luminosity), int(npackets), _p(out, _fp)))
         return out
 
 

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/cmi.h"
+#include "cmi_ctx.hpp"
 #include "cmi_device.hpp"
 
 namespace cmi {
@@ -273,12 +274,72 @@ __global__ void features_kernel(const float *__restrict__ feat, int D, int H, in
 
 __global__ void crop_target_kernel(const float *__restrict__ t, int D, int H, int W,
                                    int od, int oh, int ow, int S0, int S1, int S2,
-                                   float *__restrict__ out) {
+                                   float inv_scale, float *__restrict__ out) {
   const size_t n = (size_t)S0 * S1 * S2;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (size_t)gridDim.x * blockDim.x) {
     const int z = (int)(v % S2), y = (int)((v / S2) % S1), xx = (int)(v / ((size_t)S1 * S2));
-    out[v] = t[(((size_t)(od + xx) * H) + (oh + y)) * W + (ow + z)];
+    out[v] = t[(((size_t)(od + xx) * H) + (oh + y)) * W + (ow + z)] * inv_scale;
+  }
+}
+
+// Lattice cell (gx, gy, gz) of a context -> its device address (subgrids in
+// the reference's order (sx * nsub_y + sy) * nsub_z + sz, tile-major inside).
+__device__ __forceinline__ size_t lattice_addr(const GridDev &g, int gx, int gy, int gz) {
+  const int sx = gx / g.sc[0], sy = gy / g.sc[1], sz = gz / g.sc[2];
+  const int sub = (sx * g.nsub[1] + sy) * g.nsub[2] + sz;
+  return cell_addr(g, sub, gx - sx * g.sc[0], gy - sy * g.sc[1], gz - sz * g.sc[2]);
+}
+
+constexpr double kJHFloor = 1e-12;  // s^-1, added before the logarithm
+
+// Features of a context's cells in lattice order: log10 n_H and
+// log10(jconv * J + floor); normalised BF16 into act (channels 0, 1 of 32)
+// and/or raw FP32 into feat ([v][2]).
+__global__ void ctx_features_kernel(GridDev g, const double *__restrict__ n,
+                                    const double *__restrict__ J, int D, int H, int W,
+                                    double jconv, float m0, float s0, float m1, float s1,
+                                    __bf16 *__restrict__ act, float *__restrict__ feat) {
+  const size_t nv = (size_t)D * H * W;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int gz = (int)(v % W), gy = (int)((v / W) % H), gx = (int)(v / ((size_t)W * H));
+    const size_t a = lattice_addr(g, gx, gy, gz);
+    const float f0 = (float)log10(n[a]);
+    const float f1 = (float)log10(jconv * J[a] + kJHFloor);
+    if (feat) {
+      feat[2 * v] = f0;
+      feat[2 * v + 1] = f1;
+    }
+    if (act) {
+      bf16x8 p0 = bf16x8{}, pz = bf16x8{};
+      p0[0] = f2bf((f0 - m0) * s0);
+      p0[1] = f2bf((f1 - m1) * s1);
+      bf16x8 *o = reinterpret_cast<bf16x8 *>(act + v * kC);
+      o[0] = p0;
+      o[1] = pz;
+      o[2] = pz;
+      o[3] = pz;
+    }
+  }
+}
+
+// Logits (lattice order) -> x_n of the context, J = 0, FP32 opacity.
+__global__ void ctx_apply_kernel(GridDev g, const float *__restrict__ z, int D, int H, int W,
+                                 float out_scale, const double *__restrict__ n,
+                                 double *__restrict__ xn, double *__restrict__ J,
+                                 float *__restrict__ kappa, double kscale) {
+  const size_t nv = (size_t)D * H * W;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int gz = (int)(v % W), gy = (int)((v / W) % H), gx = (int)(v / ((size_t)W * H));
+    const size_t a = lattice_addr(g, gx, gy, gz);
+    const double logit = (double)z[v] * (double)out_scale;
+    double x = 1. / (1. + exp(-logit));
+    x = fmin(fmax(x, 1e-7), 1.);
+    xn[a] = x;
+    J[a] = 0.;
+    if (kappa) kappa[a] = (float)(kscale * n[a] * x);
   }
 }
 
@@ -312,6 +373,7 @@ struct cmi_net {
   int step = 0;
   // normalisation of the features and of the target logits
   float mean[2] = {8.f, -10.f}, scale[2] = {1.f, 0.25f};
+  float out_scale = 8.f;  // logit = out_scale * network output
   // dataset (device): features [S][N][2] FP32, targets [S][N] FP32
   float *ds_feat = nullptr, *ds_tgt = nullptr;
   int ds_n = 0, ds_dim[3] = {0, 0, 0};
@@ -373,7 +435,7 @@ unsigned grid_for(size_t items, unsigned per_block, unsigned cap = 65535) {
 // forward over act[0] (nvox voxels in B samples of D x H x W) -> z
 int forward(cmi_net *n, int B, int D, int H, int W, float *zout) {
   const int nrun = B * D * H * (W / 16);
-  const unsigned blocks = grid_for((size_t)nrun, 4);
+  const unsigned blocks = grid_for((size_t)nrun, 4, 1u << 30);  // one wave per run
   for (int l = 0; l < kLayers; ++l) {
     const float *bias = n->p + n->off_b[l];
     if (l < kLayers - 1) {
@@ -394,7 +456,7 @@ int forward(cmi_net *n, int B, int D, int H, int W, float *zout) {
 int backward(cmi_net *n, int B, int D, int H, int W) {
   const size_t nvox = (size_t)B * D * H * W;
   const int nrun = B * D * H * (W / 16);
-  const unsigned blocks = grid_for((size_t)nrun, 4);
+  const unsigned blocks = grid_for((size_t)nrun, 4, 1u << 30);  // one wave per run
   const int nrow = B * D * H * (W / 32);
   const int rows_per_wave = 16;
   const int nchunks = (nrow + rows_per_wave - 1) / rows_per_wave;
@@ -407,7 +469,7 @@ int backward(cmi_net *n, int B, int D, int H, int W) {
     const int NOUT = nout_of(l);
     const int nb_out = NOUT / 16;
     const size_t waves = (size_t)kTaps * nb_out * nchunks;
-    hipLaunchKernelGGL(wgrad_kernel, dim3(grid_for(waves, 4)), dim3(256), 0, n->stream, dy,
+    hipLaunchKernelGGL(wgrad_kernel, dim3(grid_for(waves, 4, 1u << 30)), dim3(256), 0, n->stream, dy,
                        n->act[l], nb_out, nrow, rows_per_wave, D, H, W, n->g + n->off_w[l], NOUT);
     hipLaunchKernelGGL(bgrad_kernel, dim3(grid_for(nvox, 8, 2048)), dim3(256), 0, n->stream,
                        dy, nvox, layer_cout(l), n->g + n->off_b[l]);
@@ -533,8 +595,10 @@ int cmi_net_set_params(cmi_net *n, const float *in) {
   return CMI_OK;
 }
 
-int cmi_net_set_normalisation(cmi_net *n, const float mean[2], const float scale[2]) {
-  if (!n || !mean || !scale) return CMI_ERR_ARG;
+int cmi_net_set_normalisation(cmi_net *n, const float mean[2], const float scale[2],
+                              float logit_scale) {
+  if (!n || !mean || !scale || !(logit_scale > 0.f)) return CMI_ERR_ARG;
+  n->out_scale = logit_scale;
   for (int c = 0; c < 2; ++c) {
     n->mean[c] = mean[c];
     n->scale[c] = scale[c];
@@ -609,7 +673,7 @@ int cmi_net_train(cmi_net *n, int steps, int batch, const int crop[3], float lr,
       hipLaunchKernelGGL(crop_target_kernel, dim3(grid_for(cv, 256, 4096)), dim3(256), 0,
                          n->stream, n->ds_tgt + (size_t)si * nv, n->ds_dim[0], n->ds_dim[1],
                          n->ds_dim[2], od, oh, ow, crop[0], crop[1], crop[2],
-                         n->tgt + (size_t)b * cv);
+                         1.f / n->out_scale, n->tgt + (size_t)b * cv);
     }
     hipLaunchKernelGGL(zero_f32_kernel, dim3(64), dim3(256), 0, n->stream, n->g, n->nparam);
     rc = forward(n, batch, crop[0], crop[1], crop[2], n->z);
@@ -645,11 +709,13 @@ int cmi_net_loss_and_grad(cmi_net *n, const float *feat, const float *target, in
   const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
   int rc = ensure_work(n, nv * B);
   if (rc != CMI_OK) return rc;
+  std::vector<float> ts(target, target + nv * B);
+  for (float &t : ts) t /= n->out_scale;
   float *df = nullptr;
   NET_TRY(n, hipMalloc(&df, nv * B * 2 * sizeof(float)));
   hipError_t e = hipMemcpy(df, feat, nv * B * 2 * sizeof(float), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(n->tgt, target, nv * B * sizeof(float), hipMemcpyHostToDevice);
+    e = hipMemcpy(n->tgt, ts.data(), nv * B * sizeof(float), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     hipFree(df);
     return nfail(n, CMI_ERR_HIP, "loss_and_grad: copy failed");
@@ -696,6 +762,78 @@ int cmi_net_predict(cmi_net *n, const float *feat, const int dim[3], float *logi
   if (rc != CMI_OK) return rc;
   NET_TRY(n, hipMemcpyAsync(logits, n->z, nv * sizeof(float), hipMemcpyDeviceToHost, n->stream));
   NET_TRY(n, hipStreamSynchronize(n->stream));
+  for (size_t v = 0; v < nv; ++v) logits[v] *= n->out_scale;
+  return CMI_OK;
+}
+
+namespace {
+
+// jH = jconv * J of a context's stored intensity (J is kept in units of
+// sigma * Delta_0 in FP32 contexts)
+double context_jconv(const cmi_ctx *c, double luminosity, uint64_t npackets) {
+  double j = luminosity / (double)npackets * c->g.inv_cell_size[0] * c->g.inv_cell_size[1] *
+             c->g.inv_cell_size[2];
+  if (c->precision != CMI_PREC_FP64) j *= c->sigma * c->g.cell_size[0];
+  return j;
+}
+
+int check_context(cmi_net *n, const cmi_ctx *c, double luminosity, uint64_t npackets) {
+  if (!c || npackets == 0 || !(luminosity > 0.))
+    return nfail(n, CMI_ERR_ARG, "bad warm-start arguments");
+  if (c->device != n->device) return nfail(n, CMI_ERR_ARG, "context on another device");
+  return CMI_OK;
+}
+
+}  // namespace
+
+int cmi_net_warm_start(cmi_net *n, cmi_ctx *c, double luminosity, uint64_t npackets) {
+  if (!n) return CMI_ERR_ARG;
+  int rc = check_context(n, c, luminosity, npackets);
+  if (rc != CMI_OK) return rc;
+  const int D = c->ncell[0], H = c->ncell[1], W = c->ncell[2];
+  if (W % 16 != 0) return nfail(n, CMI_ERR_ARG, "ncell[2] must be a multiple of 16");
+  (void)hipSetDevice(n->device);
+  const size_t nv = (size_t)D * H * W;
+  rc = ensure_work(n, nv);
+  if (rc != CMI_OK) return rc;
+  NET_TRY(n, hipStreamSynchronize(c->stream));  // the first pass has finished
+  hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0,
+                     n->stream, c->g, (const double *)c->f.n, (const double *)c->f.J, D, H, W,
+                     context_jconv(c, luminosity, npackets), n->mean[0], n->scale[0],
+                     n->mean[1], n->scale[1], n->act[0], (float *)nullptr);
+  rc = forward(n, 1, D, H, W, n->z);
+  if (rc != CMI_OK) return rc;
+  const bool fp32 = c->precision == CMI_PREC_FP32;
+  hipLaunchKernelGGL(ctx_apply_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, n->stream,
+                     c->g, (const float *)n->z, D, H, W, n->out_scale, (const double *)c->f.n,
+                     c->f.xn, c->f.J, fp32 ? c->f.kappa : (float *)nullptr,
+                     c->sigma * c->g.cell_size[0]);
+  NET_TRY(n, hipGetLastError());
+  NET_TRY(n, hipStreamSynchronize(n->stream));
+  return CMI_OK;
+}
+
+int cmi_net_context_features(cmi_net *n, cmi_ctx *c, double luminosity, uint64_t npackets,
+                             float *features) {
+  if (!n || !features) return CMI_ERR_ARG;
+  int rc = check_context(n, c, luminosity, npackets);
+  if (rc != CMI_OK) return rc;
+  (void)hipSetDevice(n->device);
+  const int D = c->ncell[0], H = c->ncell[1], W = c->ncell[2];
+  const size_t nv = (size_t)D * H * W;
+  float *d = nullptr;
+  NET_TRY(n, hipStreamSynchronize(c->stream));
+  NET_TRY(n, hipMalloc(&d, nv * 2 * sizeof(float)));
+  hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0,
+                     n->stream, c->g, (const double *)c->f.n, (const double *)c->f.J, D, H, W,
+                     context_jconv(c, luminosity, npackets), 0.f, 1.f, 0.f, 1.f,
+                     (__bf16 *)nullptr, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(features, d, nv * 2 * sizeof(float), hipMemcpyDeviceToHost, n->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(n->stream);
+  hipFree(d);
+  if (e != hipSuccess) return nfail(n, CMI_ERR_HIP, std::string("context features: ") + hipGetErrorString(e));
   return CMI_OK;
 }
 

@@ -1,0 +1,188 @@
+"""Learned warm start of the ionization iteration.
+
+The plain method (testDensitySubGrid_omp.cpp:2274-2434) starts every cell at
+x_n = 1e-6 (DensitySubGrid.hpp:549) and alternates a photon transport with the
+ionization-equilibrium update until the neutral fraction stops changing.  The
+warm start replaces the starting x_n only: a cheap first transport (1e6
+packets through the x_n = 1e-6 medium) gives the unattenuated
+photoionization rate jH per cell, the network maps (log10 n_H, log10 jH) to
+the logit of the converged x_n, and the plain iteration then runs from there
+and still decides convergence (mean |x_n change| per cell < tol).
+
+Training data are fields the method itself converges: clumpy log-normal media
+at the 256^3 clumpy workload's cell size, cut into 64^3 crops (features: the
+first-pass (log10 n_H, log10 jH); target: logit of the converged x_n).
+"""
+import time
+
+import numpy as np
+
+from . import (ENGINE_TILES, PREC_FP32, Context, WarmStartNet)
+from . import workloads as W
+
+FIRST_PASS_PACKETS = 1_000_000
+XN_CLAMP = 1e-7          # targets use logit of x_n clamped to [1e-7, 1 - 1e-7]
+LOGIT_SCALE = 8.0        # the network outputs logit / 8
+FEATURE_MEAN = (8.0, -7.0)
+FEATURE_SCALE = (2.0, 0.5)
+
+
+def logit(x):
+    x = np.clip(np.asarray(x, dtype=np.float64), XN_CLAMP, 1.0 - XN_CLAMP)
+    return np.log(x / (1.0 - x))
+
+
+def clumpy_problem(ncell=256, box_side_pc=10.0, nsrc=8, field_seed=2,
+                   src_seed=3, lum_seed=4, sigma_ln=1.0, lmin=1e48, lmax=1e49):
+    """Log-normal medium (P(k) ~ k^-11/3, mean 1e8 m^-3) with point sources
+    at uniform-random positions and log-uniform luminosities.  The defaults
+    are the 256^3 clumpy workload; returns (box, n_H lattice, pos, L)."""
+    half = 0.5 * box_side_pc * W.PC
+    box = np.array([-half, -half, -half, 2 * half, 2 * half, 2 * half])
+    n = W.lognormal_density(ncell, sigma_ln=sigma_ln, seed=field_seed)
+    pos, L = W.random_sources(nsrc, box, seed=src_seed, lum_seed=lum_seed,
+                              lmin=lmin, lmax=lmax)
+    return box, n, pos, L
+
+
+class Problem:
+    """A FP32-traversal context on the tiles engine holding one medium, with
+    its density kept for resets."""
+
+    def __init__(self, box, n_lattice, pos, L, nsub=4, alpha=W.ALPHA_B):
+        ncell = list(n_lattice.shape)
+        self.ncell = ncell
+        self.nsub = [nsub] * 3
+        self.ctx = Context(box, ncell, self.nsub, PREC_FP32)
+        self.ctx.set_engine(ENGINE_TILES)
+        self.n_ref = W.global_to_subgrid_major(n_lattice, self.nsub)
+        self.xn0 = np.full(self.n_ref.size, W.XN0)
+        self.ctx.set_sources(pos, L)
+        self.L = self.ctx.total_luminosity()
+        self.alpha = alpha
+        self.reset()
+
+    def reset(self):
+        """x_n = 1e-6 everywhere, J = 0."""
+        self.ctx.set_fields(self.n_ref, self.xn0)
+        self.ctx.reset_intensity()
+        self.ctx.synchronize()
+
+    def xn_lattice(self):
+        _, x = self.ctx.get_fields()
+        return W.subgrid_major_to_global(x, self.ncell, self.nsub)
+
+    def first_pass(self, seed, npackets=FIRST_PASS_PACKETS):
+        """One transport through the current medium; leaves J in place."""
+        return self.ctx.transport(npackets, seed, 0)
+
+    def iterate(self, npackets, seed, tol=1e-3, max_it=40, it0=1,
+                min_it=1):
+        """Plain iterations (transport + FP64 update) until the mean
+        |x_n change| < tol; returns the per-iteration L1 changes and the wall
+        time (host-synchronised)."""
+        l1s = []
+        t0 = time.perf_counter()
+        for it in range(max_it):
+            self.ctx.transport(npackets, seed, it0 + it)
+            l1 = self.ctx.update_ionization(self.L, npackets, self.alpha, form=1)
+            l1s.append(l1)
+            if it + 1 >= min_it and l1 < tol:
+                break
+        self.ctx.synchronize()
+        return {"l1": l1s, "iterations": len(l1s),
+                "seconds": time.perf_counter() - t0,
+                "converged": l1s[-1] < tol}
+
+    def close(self):
+        self.ctx.close()
+
+
+def converge_plain(prob, npackets, seed, tol=1e-3, max_it=40):
+    prob.reset()
+    r = prob.iterate(npackets, seed, tol=tol, max_it=max_it)
+    r["total_seconds"] = r["seconds"]
+    return r
+
+
+def converge_warm(prob, net, npackets, seed, tol=1e-3, max_it=40,
+                  first_packets=FIRST_PASS_PACKETS):
+    """First pass + network + plain iterations; the wall time includes the
+    first pass and the inference."""
+    prob.reset()
+    t0 = time.perf_counter()
+    prob.first_pass(seed ^ 0x5A5A5A5A, first_packets)
+    net.warm_start(prob.ctx, prob.L, first_packets)
+    t1 = time.perf_counter()
+    r = prob.iterate(npackets, seed, tol=tol, max_it=max_it)
+    r["warm_seconds"] = t1 - t0
+    r["total_seconds"] = r["seconds"] + (t1 - t0)
+    return r
+
+
+def crops(a, size):
+    """All non-overlapping size^3 crops of a lattice (leading axes)."""
+    n0, n1, n2 = a.shape[:3]
+    out = []
+    for i in range(0, n0 - size + 1, size):
+        for j in range(0, n1 - size + 1, size):
+            for k in range(0, n2 - size + 1, size):
+                out.append(a[i:i + size, j:j + size, k:k + size])
+    return out
+
+
+def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
+                      nsrc=8, tol=1e-3, max_it=40, extra_it=0, log=None,
+                      feature_net=None):
+    """Converge one clumpy problem per seed with the plain method and cut
+    (features, target logits) crops.  Returns float32 arrays
+    (S, crop, crop, crop, 2) and (S, crop, crop, crop) plus per-problem
+    records."""
+    net = feature_net or WarmStartNet(seed=1)
+    feats, tgts, recs = [], [], []
+    for s in seeds:
+        box, n, pos, L = clumpy_problem(ncell, box_side_pc, nsrc,
+                                        field_seed=1000 + s, src_seed=2000 + s,
+                                        lum_seed=3000 + s)
+        prob = Problem(box, n, pos, L)
+        prob.first_pass(seed=7000 + s)
+        f = net.context_features(prob.ctx, prob.L, FIRST_PASS_PACKETS)
+        prob.reset()
+        r = prob.iterate(npackets, seed=5000 + s, tol=tol, max_it=max_it)
+        if extra_it:
+            prob.iterate(npackets, seed=6000 + s, tol=0.0, max_it=extra_it,
+                         it0=100)
+        t = logit(prob.xn_lattice()).astype(np.float32)
+        feats += crops(f, crop)
+        tgts += crops(t, crop)
+        recs.append({"seed": int(s), "iterations": r["iterations"],
+                     "l1": r["l1"][-1]})
+        if log:
+            log("training problem %d: %d iterations, last L1 %.2e" %
+                (s, r["iterations"], r["l1"][-1]))
+        prob.close()
+    if feature_net is None:
+        net.close()
+    return (np.ascontiguousarray(np.stack(feats)),
+            np.ascontiguousarray(np.stack(tgts)), recs)
+
+
+def new_net(seed=1):
+    net = WarmStartNet(seed=seed)
+    net.set_normalisation(FEATURE_MEAN, FEATURE_SCALE, LOGIT_SCALE)
+    return net
+
+
+def save_net(net, path, meta=None):
+    np.savez(path, params=net.params(), feature_mean=np.array(FEATURE_MEAN),
+             feature_scale=np.array(FEATURE_SCALE), logit_scale=LOGIT_SCALE,
+             meta=np.array(repr(meta or {})))
+
+
+def load_net(path):
+    d = np.load(path, allow_pickle=False)
+    net = WarmStartNet(seed=1)
+    net.set_normalisation(d["feature_mean"], d["feature_scale"],
+                          float(d["logit_scale"]))
+    net.set_params(d["params"])
+    return net

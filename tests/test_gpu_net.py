@@ -66,7 +66,7 @@ def torch_forward(params, feat):
 @pytest.fixture(scope="module")
 def net():
     n = cmi.WarmStartNet(seed=3)
-    n.set_normalisation([0.0, 0.0], [1.0, 1.0])
+    n.set_normalisation([0.0, 0.0], [1.0, 1.0], 1.0)
     yield n
     n.close()
 
@@ -121,7 +121,7 @@ def test_training_reduces_loss():
     f = (f / f.std()).astype(np.float32)
     t = (2.0 * np.tanh(f[..., 0]) - 1.5 * f[..., 1] ** 2).astype(np.float32)
     n = cmi.WarmStartNet(seed=5)
-    n.set_normalisation([0.0, 0.0], [1.0, 1.0])
+    n.set_normalisation([0.0, 0.0], [1.0, 1.0], 1.0)
     n.add_samples(f, t)
     losses = n.train(300, 2, [32, 32, 32], 2e-3, seed=1)
     n.close()
