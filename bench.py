@@ -117,6 +117,77 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_bytes": nbytes, "copies": 2}))
 
 
+def warm_start_section(args, box, ncell, nsub, n_ref, pos, L):
+    """Wall time to a converged x_n (mean |change| < tol) from x_n = 1e-6:
+    the plain iteration against the warm start (first pass of 1e6 packets +
+    network inference, then the same plain iteration).  Both run the same
+    packets per iteration on the same context type (FP32 tiles engine)."""
+    from parallelcmacionize_amd import warmstart as WS
+    from parallelcmacionize_amd import workloads as W
+    if not os.path.exists(args.net):
+        return {"skipped": "no trained network at %s" % args.net}
+    N = int(args.packets)
+    n_lat = W.subgrid_major_to_global(n_ref, [ncell] * 3, nsub)
+    prob = WS.Problem(box, n_lat, pos, L, nsub=nsub[0])
+    net = WS.load_net(args.net)
+    # one untimed pass of each to build the engines' buffers and graphs
+    prob.first_pass(seed=99)
+    net.warm_start(prob.ctx, prob.L, WS.FIRST_PASS_PACKETS)
+    plain = WS.converge_plain(prob, N, seed=21, tol=args.tol)
+    x_plain = prob.xn_lattice()
+    warm = WS.converge_warm(prob, net, N, seed=21, tol=args.tol)
+    x_warm = prob.xn_lattice()
+    r_plain = W.ionized_radius(x_plain, box, [ncell] * 3)
+    r_warm = W.ionized_radius(x_warm, box, [ncell] * 3)
+    net.close()
+    prob.close()
+    log("plain: %d iterations %.3f s; warm: %d iterations %.3f s"
+        % (plain["iterations"], plain["total_seconds"], warm["iterations"],
+           warm["total_seconds"]))
+    return {
+        "tol_l1": args.tol,
+        "packets_per_iteration": N,
+        "plain_iterations": plain["iterations"],
+        "plain_seconds": plain["total_seconds"],
+        "plain_l1": plain["l1"],
+        "warm_iterations": warm["iterations"],
+        "warm_seconds": warm["total_seconds"],
+        "warm_first_pass_and_inference_seconds": warm["warm_seconds"],
+        "warm_l1": warm["l1"],
+        "wall_time_ratio_plain_over_warm": plain["total_seconds"] / warm["total_seconds"],
+        "converged_l1_warm_vs_plain": float(np.mean(np.abs(x_warm - x_plain))),
+        "ionized_radius_plain_m": r_plain,
+        "ionized_radius_warm_m": r_warm,
+        "ionized_radius_rel_diff": abs(r_warm / r_plain - 1.0),
+        "network": os.path.relpath(args.net, ROOT),
+    }
+
+
+def batched_section(args):
+    """64 problems of uneven size in one graph-captured problem-set
+    iteration: throughput and per-problem completion times."""
+    from parallelcmacionize_amd import batched as B
+    from parallelcmacionize_amd import autotune as A
+    spec = B.problem_set_spec()
+    fields = B.build_fields(spec)
+    cache = A.load_cache(os.path.join(ROOT, "parallelcmacionize_amd", "data",
+                                      "tuning.json"))
+    cfg = cache.get("batched64", dict(A.DEFAULT))
+    ps = B.ProblemSet(spec, fields, tile=cfg["tile"],
+                      block_threads=cfg["block_threads"],
+                      refill_batch=cfg["refill_batch"], chunk=cfg["chunk"])
+    B.run(ps, 2, seed=1, it0=0)                     # warm-up, graph capture
+    r = B.run(ps, args.batched_iterations, seed=1, it0=2)
+    ps.close()
+    r["tuning"] = cfg
+    r["tuning_source"] = ("autotuner cache" if "batched64" in cache
+                          else "defaults")
+    log("batched: %.3e crossings/s, p50 %.4f s, p99 %.4f s"
+        % (r["crossings_per_s"], r["problem_seconds_p50"],
+           r["problem_seconds_p99"]))
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +200,13 @@ def main():
     ap.add_argument("--nsub", type=int, default=4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--net", default=os.path.join(
+        ROOT, "parallelcmacionize_amd", "data", "warmstart_clumpy256.npz"),
+        help="trained warm-start network for the plain-vs-warm comparison")
+    ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-iterations", type=int, default=5)
+    ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--copy-calibration", type=float, default=0,
                     help="only copy this many bytes device-to-device twice "
                          "(reference kernel for the HBM traffic counters)")
@@ -243,6 +321,10 @@ def main():
             args.warmup + args.steps, args.cpu_seconds, nthreads)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     ctx.close()
+    if not args.no_warm:
+        out["warm_start"] = warm_start_section(args, box, ncell, nsub, n, pos, L)
+    if not args.no_batched:
+        out["batched"] = batched_section(args)
     print(json.dumps(out))
 
 

@@ -231,6 +231,9 @@ int cmi_batch_set_fields(cmi_batch *b, int problem, const double *nH,
 int cmi_batch_get_fields(cmi_batch *b, int problem, double *J, double *xn);
 int cmi_batch_set_sources(cmi_batch *b, int problem, const double *pos,
                           const double *L, int nsrc);
+/* Change the launch knobs of a set (threads, refill batch, chunk, LDS
+ * accumulator); the tile edge is fixed at creation and must be 0 or equal. */
+int cmi_batch_set_tuning(cmi_batch *b, const cmi_tuning *tuning);
 /* One iteration of every problem: npackets[p] packets for problem p, then the
  * FP64 ionization update (alphaH, form as cmi_update_ionization).  Per
  * problem (arrays of nprob, any may be NULL): mean |dx_n|, the time from the

@@ -104,6 +104,7 @@ SIGNATURES = {
     "cmi_batch_set_fields": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "cmi_batch_get_fields": (C.c_int, [_vp, C.c_int, _dp, _dp]),
     "cmi_batch_set_sources": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int]),
+    "cmi_batch_set_tuning": (C.c_int, [_vp, C.POINTER(Tuning)]),
     "cmi_batch_iteration": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint32,
                                       C.c_double, C.c_int, _dp, _dp, _u64p,
                                       C.POINTER(Stats)]),
@@ -384,6 +385,15 @@ class Batch:
         l = np.ascontiguousarray(L, dtype=np.float64).reshape(-1)
         self._check(self.lib.cmi_batch_set_sources(self.h, int(p), _p(a, _dp),
                                                    _p(l, _dp), len(l)))
+
+    def set_tuning(self, block_threads=0, refill_batch=0, chunk=0,
+                   lds_fp32=0):
+        t = Tuning()
+        t.block_threads = int(block_threads)
+        t.refill_batch = int(refill_batch)
+        t.chunk_min = int(chunk)
+        t.lds_fp32 = int(lds_fp32)
+        self._check(self.lib.cmi_batch_set_tuning(self.h, C.byref(t)))
 
     def iteration(self, npackets, seed, iteration, alpha, form=1):
         n = np.ascontiguousarray(npackets, dtype=np.uint64)
