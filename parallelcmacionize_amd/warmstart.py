@@ -13,6 +13,7 @@ Training data are fields the method itself converges: clumpy log-normal media
 at the 256^3 clumpy workload's cell size, cut into 64^3 crops (features: the
 first-pass (log10 n_H, log10 jH); target: logit of the converged x_n).
 """
+import json
 import time
 
 import numpy as np
@@ -167,16 +168,18 @@ def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
             np.ascontiguousarray(np.stack(tgts)), recs)
 
 
-def new_net(seed=1):
+def new_net(seed=1, mean=FEATURE_MEAN, scale=FEATURE_SCALE):
     net = WarmStartNet(seed=seed)
-    net.set_normalisation(FEATURE_MEAN, FEATURE_SCALE, LOGIT_SCALE)
+    net.set_normalisation(mean, scale, LOGIT_SCALE)
+    net.norm = (np.asarray(mean, np.float32), np.asarray(scale, np.float32))
     return net
 
 
 def save_net(net, path, meta=None):
-    np.savez(path, params=net.params(), feature_mean=np.array(FEATURE_MEAN),
-             feature_scale=np.array(FEATURE_SCALE), logit_scale=LOGIT_SCALE,
-             meta=np.array(repr(meta or {})))
+    mean, scale = getattr(net, "norm", (FEATURE_MEAN, FEATURE_SCALE))
+    np.savez(path, params=net.params(), feature_mean=np.asarray(mean),
+             feature_scale=np.asarray(scale), logit_scale=LOGIT_SCALE,
+             meta=np.array(json.dumps(meta or {})))
 
 
 def load_net(path):

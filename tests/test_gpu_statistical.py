@@ -1,0 +1,118 @@
+"""Statistical oracle: the converged neutral fraction of the uniform
+Stromgren sphere (64^3 cells in [-5 pc, 5 pc]^3 as 4^3 subgrids, n_H = 1e8
+m^-3, one central source of 4.26e49 s^-1, 1e6 packets per iteration, 10
+iterations from x_n = 1e-6) from the GPU production path (FP32 tiles engine,
+FP64 update) against the reference's own transport and update
+(DensitySubGrid::interact under its task-parallel buffer routing, and
+DensitySubGrid::compute_neutral_fraction with its hard-coded alpha = 4e-19,
+built from the reference sources in oracle/_ref) on the host cores.
+
+The two legs use independent packet streams (GPU seeds 1-10, CPU seeds
+101-110).  Per radial bin (width 0.05 R_s) the seed-means of x_n must agree
+within 3 sigma of the seed-to-seed scatter, sigma^2 = var_gpu / 10 +
+var_cpu / 10, in at least 90 % of the bins and within 4.5 sigma in all
+(26 bins at 3 sigma would fail one bin in 14 runs by chance alone); both
+legs must find the analytic Stromgren radius within 2 %."""
+import os
+
+import numpy as np
+import pytest
+
+import parallelcmacionize_amd as cmi
+from parallelcmacionize_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+NC = 64
+NSUB = [4, 4, 4]
+N = 1_000_000
+ITER = 10
+SEEDS = 10
+ALPHA = W.ALPHA_REF   # the reference update's own alpha
+
+
+def radial_bins(Rs):
+    box = W.box_10pc()
+    X, Y, Z = W.cell_centres(box, [NC] * 3)
+    r = np.sqrt(X * X + Y * Y + Z * Z) / Rs
+    edges = np.arange(0.0, 1.31, 0.05)
+    idx = np.digitize(r, edges) - 1
+    return W.global_to_subgrid_major(idx, NSUB), len(edges) - 1
+
+
+def bin_means(x, idx, nb):
+    s = np.bincount(idx, weights=x, minlength=nb + 1)[:nb]
+    c = np.bincount(idx, minlength=nb + 1)[:nb]
+    return s / np.maximum(c, 1), c
+
+
+def gpu_leg(seed):
+    box = W.box_10pc()
+    with cmi.Context(box, [NC] * 3, NSUB, cmi.PREC_FP32) as ctx:
+        ctx.set_engine(cmi.ENGINE_TILES)
+        ctx.set_fields(np.full(NC ** 3, W.N_H), np.full(NC ** 3, W.XN0))
+        ctx.set_sources([[0.0, 0.0, 0.0]], [W.Q_STROMGREN])
+        for it in range(ITER):
+            ctx.transport(N, seed, it)
+            ctx.update_ionization(W.Q_STROMGREN, N, ALPHA, form=0)
+        return ctx.get_fields()[1]
+
+
+def cpu_leg(O, lib, ref, seed, nthreads):
+    box = W.box_10pc()
+    og = O.OracleGrid(lib, box, [NC] * 3, NSUB)   # Philox emission only
+    n = np.full(NC ** 3, W.N_H)
+    xn = np.full(NC ** 3, W.XN0)
+    b = np.ascontiguousarray(box)
+    nc = np.array([NC] * 3, np.int32)
+    ns = np.array(NSUB, np.int32)
+    for it in range(ITER):
+        pk, ss = og.emit([[0.0, 0.0, 0.0]], [W.Q_STROMGREN], N, seed, it)
+        J = np.zeros(NC ** 3)
+        ref.ref_transport_tasks(O.ptr(b, O._dp), O.ptr(nc, O._ip),
+                                O.ptr(ns, O._ip), O.ptr(n, O._dp),
+                                O.ptr(xn, O._dp), O.ptr(J, O._dp),
+                                pk.ctypes.data, O.ptr(ss, O._u32p), N,
+                                nthreads, 0)
+        # cells are independent in the update: the whole grid as one block
+        # with the same cell size
+        out = np.zeros_like(xn)
+        ref.ref_compute_neutral_fraction(O.ptr(b, O._dp), O.ptr(nc, O._ip),
+                                         O.ptr(n, O._dp), O.ptr(J, O._dp),
+                                         W.Q_STROMGREN, N, O.ptr(out, O._dp))
+        xn = out
+    return xn
+
+
+def test_converged_field_matches_reference_within_3_sigma(oracle):
+    import oracle_lib as O
+    ref = O.load_ref(fast=False)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (needs the reference checkout)")
+    nthreads = min(16, os.cpu_count() or 1)
+    Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, ALPHA)
+    idx, nb = radial_bins(Rs)
+    g = np.array([bin_means(gpu_leg(s), idx, nb)[0]
+                  for s in range(1, SEEDS + 1)])
+    xs = [cpu_leg(O, oracle, ref, 100 + s, nthreads) for s in range(1, SEEDS + 1)]
+    c = np.array([bin_means(x, idx, nb)[0] for x in xs])
+    counts = bin_means(xs[0], idx, nb)[1]
+    use = counts > 0
+    dg, dc = g.mean(0), c.mean(0)
+    sig = np.sqrt(g.var(0, ddof=1) / SEEDS + c.var(0, ddof=1) / SEEDS)
+    # floor: bins where both legs are deterministic to rounding
+    sig = np.maximum(sig, 1e-9 * np.maximum(np.abs(dg), np.abs(dc)) + 1e-15)
+    z = np.abs(dg - dc)[use] / sig[use]
+    print("bins", int(use.sum()), "max z %.2f" % z.max(), "mean z %.2f" % z.mean(),
+          "frac within 3 sigma %.3f" % np.mean(z <= 3))
+    for k in np.nonzero(use)[0]:
+        print("r/Rs %.2f  gpu %.4e  cpu %.4e  sigma %.2e" % (0.05 * k + 0.025,
+              dg[k], dc[k], sig[k]))
+    assert np.mean(z <= 3) >= 0.9
+    assert z.max() <= 4.5
+    box = W.box_10pc()
+    Rg = np.mean([W.ionized_radius(gpu_leg(s), box, [NC] * 3) for s in (1, 2)])
+    Rc = np.mean([W.ionized_radius(x, box, [NC] * 3) for x in xs[:2]])
+    print("R/Rs gpu %.4f cpu %.4f" % (Rg / Rs, Rc / Rs))
+    assert abs(Rg / Rs - 1) < 0.02
+    assert abs(Rc / Rs - 1) < 0.02

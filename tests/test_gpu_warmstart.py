@@ -1,0 +1,95 @@
+"""GPU checks of the learned warm start: the device-side features and
+x_n write-back against host computations on the same context, and the gain
+of the trained network over the plain iteration on the 256^3 clumpy
+workload (wall time to mean |dx_n| < 1e-3 from x_n = 1e-6, same packets per
+iteration, equal converged field and ionized radius)."""
+import os
+
+import numpy as np
+import pytest
+
+import parallelcmacionize_amd as cmi
+from parallelcmacionize_amd import warmstart as WS
+from parallelcmacionize_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+NET = os.path.join(os.path.dirname(cmi.__file__), "data",
+                   "warmstart_clumpy256.npz")
+
+
+@pytest.fixture(scope="module")
+def small():
+    box, n, pos, L = WS.clumpy_problem(64, 2.5, 2, field_seed=41, src_seed=42,
+                                       lum_seed=43, lmin=1e46, lmax=1e47)
+    prob = WS.Problem(box, n, pos, L)
+    net = WS.new_net(5)
+    yield prob, net
+    net.close()
+    prob.close()
+
+
+def host_features(prob, npackets):
+    J, _ = prob.ctx.get_fields()
+    J = W.subgrid_major_to_global(J, prob.ncell, prob.nsub)
+    cell = prob.ctx.box[3] / prob.ncell[0]
+    jH = prob.L / npackets * J / cell ** 3
+    n = W.subgrid_major_to_global(prob.n_ref, prob.ncell, prob.nsub)
+    return np.stack([np.log10(n), np.log10(jH + 1e-12)], axis=-1)
+
+
+def test_context_features_match_host(small):
+    prob, net = small
+    prob.reset()
+    N = 200000
+    prob.first_pass(seed=3, npackets=N)
+    f = net.context_features(prob.ctx, prob.L, N)
+    h = host_features(prob, N)
+    assert f.shape == h.shape
+    assert np.abs(f - h).max() < 2e-5
+    assert (h[..., 1] > -11).mean() > 0.5   # most cells saw packets
+
+
+def test_warm_start_writes_network_prediction(small):
+    prob, net = small
+    prob.reset()
+    N = 200000
+    prob.first_pass(seed=4, npackets=N)
+    f = net.context_features(prob.ctx, prob.L, N)
+    z = net.predict(f).astype(np.float64)
+    x_expect = np.clip(1.0 / (1.0 + np.exp(-z)), 1e-7, 1.0)
+    net.warm_start(prob.ctx, prob.L, N)
+    J, x = prob.ctx.get_fields()
+    x = W.subgrid_major_to_global(x, prob.ncell, prob.nsub)
+    assert np.all(J == 0)
+    assert np.allclose(x, x_expect, rtol=1e-5, atol=1e-12)
+    # the opacity follows the new x_n: a transport now sees it
+    s = prob.ctx.transport(N, 5, 1)
+    assert s["errors"] == 0
+
+
+@pytest.mark.skipif(not os.path.exists(NET), reason="no trained network file")
+def test_trained_network_halves_wall_time_on_256_clumpy():
+    box, n, pos, L = WS.clumpy_problem()          # the 256^3 clumpy workload
+    prob = WS.Problem(box, n, pos, L)
+    net = WS.load_net(NET)
+    N = 100_000_000
+    prob.first_pass(seed=99)                      # builds the engine buffers
+    net.warm_start(prob.ctx, prob.L, WS.FIRST_PASS_PACKETS)
+    plain = WS.converge_plain(prob, N, seed=21)
+    x_plain = prob.xn_lattice()
+    warm = WS.converge_warm(prob, net, N, seed=21)
+    x_warm = prob.xn_lattice()
+    net.close()
+    prob.close()
+    ratio = plain["total_seconds"] / warm["total_seconds"]
+    l1 = float(np.mean(np.abs(x_warm - x_plain)))
+    r_p = W.ionized_radius(x_plain, box, [256] * 3)
+    r_w = W.ionized_radius(x_warm, box, [256] * 3)
+    print("plain", plain["iterations"], plain["total_seconds"], "warm",
+          warm["iterations"], warm["total_seconds"], "ratio", ratio,
+          "L1", l1, "R", r_p, r_w)
+    assert plain["converged"] and warm["converged"]
+    assert warm["iterations"] < plain["iterations"]
+    assert ratio >= 1.5
+    assert l1 < 1e-3
+    assert abs(r_w / r_p - 1) < 0.02
