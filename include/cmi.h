@@ -212,6 +212,22 @@ int cmi_kernel_timing(cmi_ctx *ctx, double *avg_seconds, uint64_t *calls,
 /* Synchronise the context's stream. */
 int cmi_synchronize(cmi_ctx *ctx);
 
+/* ---- result writers (the reference's on-disk formats) ----
+ * neutral_fractions.dat: output_neutral_fractions, testDensitySubGrid_omp.cpp:
+ * 394-440, with DensitySubGrid::output_intensities, DensitySubGrid.hpp:
+ * 1738-1796 -- per subgrid, per cell (z fastest): centre x, y, z, x_n, n_H as
+ * f64, then 5 f32 hydro primitives (zero: no hydrodynamics), 60 B per cell.
+ * Arrays in the reference layout; no GPU needed. */
+int cmi_write_neutral_fractions_dat(const char *path, const double box[6],
+                                    const int ncell[3], const int nsub[3],
+                                    const double *nH, const double *xn);
+/* neutral_fractions.bin: output_neutral_fractions_binary,
+ * testDensitySubGrid_omp.cpp:507-521 -- x_n as raw f64, reference layout */
+int cmi_write_neutral_fractions_bin(const char *path, uint64_t ncell_total,
+                                    const double *xn);
+/* both files from a context's current fields (either path may be NULL) */
+int cmi_write_results(cmi_ctx *ctx, const char *dat_path, const char *bin_path);
+
 /* ---- problem sets ----
  * Many independent grids transported together (no counterpart in the
  * reference, which runs one grid per process): every problem has its own box,

@@ -18,6 +18,7 @@
  */
 #define NO_HYDRO
 #include "DensitySubGrid.hpp"
+#include "MemoryMap.hpp"
 #include "MemorySpace.hpp"
 #include "PhotonBuffer.hpp"
 #include "TravelDirections.hpp"
@@ -367,3 +368,21 @@ double ref_transport_tasks(const double box[6], const int ncell[3],
 }
 
 } /* extern "C" */
+
+/* The reference's memory-mapped result file (output_neutral_fractions,
+ * testDensitySubGrid_omp.cpp:394-440): one block of get_output_size() bytes
+ * per subgrid, written by DensitySubGrid::output_intensities
+ * (DensitySubGrid.hpp:1738-1796). */
+extern "C" size_t ref_output_dat(const double box[6], const int ncell[3],
+                                 const int nsub[3], const double *n,
+                                 const double *xn, const char *path) {
+  RefGrid g(box, ncell, nsub);
+  g.set_fields(n, xn);
+  const size_t block = g.sub[0]->get_output_size();
+  {
+    MemoryMap file(path, g.sub.size() * block);
+    for (size_t k = 0; k < g.sub.size(); ++k)
+      g.sub[k]->output_intensities(k * block, file);
+  }
+  return g.sub.size() * block;
+}

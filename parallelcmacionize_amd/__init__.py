@@ -13,7 +13,7 @@ import numpy as np
 __all__ = [
     "CMIError", "Context", "PHOTON_DTYPE", "load_library", "PREC_FP64",
     "PREC_FP32", "ENGINE_WALK", "ENGINE_TILES", "INSIDE", "output_to_input", "Batch",
-    "WarmStartNet",
+    "WarmStartNet", "write_neutral_fractions_dat", "write_neutral_fractions_bin",
     "direction_from_mask", "STATS_FIELDS",
 ]
 
@@ -108,6 +108,11 @@ SIGNATURES = {
     "cmi_batch_iteration": (C.c_int, [_vp, _u64p, C.c_uint64, C.c_uint32,
                                       C.c_double, C.c_int, _dp, _dp, _u64p,
                                       C.POINTER(Stats)]),
+    "cmi_write_neutral_fractions_dat": (C.c_int, [C.c_char_p, _dp, _ip, _ip,
+                                                  _dp, _dp]),
+    "cmi_write_neutral_fractions_bin": (C.c_int, [C.c_char_p, C.c_uint64,
+                                                  _dp]),
+    "cmi_write_results": (C.c_int, [_vp, C.c_char_p, C.c_char_p]),
     "cmi_net_create": (C.c_int, [C.c_uint64, C.POINTER(_vp)]),
     "cmi_net_destroy": (None, [_vp]),
     "cmi_net_last_error": (C.c_char_p, [_vp]),
@@ -293,6 +298,12 @@ class Context:
                                            C.byref(s)))
         return _stats_dict(s)
 
+    def write_results(self, dat_path=None, bin_path=None):
+        """neutral_fractions.dat / .bin of the current fields."""
+        self._check(self.lib.cmi_write_results(
+            self.h, None if dat_path is None else os.fsencode(dat_path),
+            None if bin_path is None else os.fsencode(bin_path)))
+
     def update_ionization(self, luminosity, npackets, alpha, form=0):
         l1 = C.c_double()
         self._check(self.lib.cmi_update_ionization(self.h, float(luminosity),
@@ -315,6 +326,34 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.cmi_synchronize(self.h))
+
+
+def write_neutral_fractions_dat(path, box, ncell, nsub, nH, xn):
+    """neutral_fractions.dat of the reference (host arrays, reference
+    layout)."""
+    lib = load_library()
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    ns = np.ascontiguousarray(nsub, dtype=np.int32)
+    n = np.ascontiguousarray(nH, dtype=np.float64).reshape(-1)
+    x = np.ascontiguousarray(xn, dtype=np.float64).reshape(-1)
+    if n.size != int(np.prod(nc)) or x.size != n.size:
+        raise ValueError("field size")
+    rc = lib.cmi_write_neutral_fractions_dat(os.fsencode(path), _p(b, _dp),
+                                             _p(nc, _ip), _p(ns, _ip),
+                                             _p(n, _dp), _p(x, _dp))
+    if rc != 0:
+        raise CMIError(rc, "cannot write %s" % path)
+
+
+def write_neutral_fractions_bin(path, xn):
+    """neutral_fractions.bin of the reference (x_n, reference layout)."""
+    lib = load_library()
+    x = np.ascontiguousarray(xn, dtype=np.float64).reshape(-1)
+    rc = lib.cmi_write_neutral_fractions_bin(os.fsencode(path), x.size,
+                                             _p(x, _dp))
+    if rc != 0:
+        raise CMIError(rc, "cannot write %s" % path)
 
 
 class Batch:

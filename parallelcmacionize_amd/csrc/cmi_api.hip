@@ -158,6 +158,19 @@ void fill_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets, double seconds,
 
 }  // namespace
 
+namespace cmi {
+// n_H of a context in the reference layout (for the result writers)
+int get_density_ref(cmi_ctx *c, double *nH) {
+  (void)hipSetDevice(c->device);
+  CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<double> d(c->ncell_total);
+  CMI_TRY_HIP(c, hipMemcpy(d.data(), c->f.n, d.size() * sizeof(double),
+                           hipMemcpyDeviceToHost));
+  from_device_order(c, d, nH);
+  return CMI_OK;
+}
+}  // namespace cmi
+
 extern "C" {
 
 int cmi_output_to_input_direction(int d) {

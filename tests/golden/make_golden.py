@@ -136,7 +136,38 @@ def main():
     np.savez_compressed(os.path.join(HERE, "neutral_fraction_16.npz"),
                         box=box, ncell=ncell, n=n, J=J, L=L, N=N, xn=xo)
     print("fixtures written to", HERE)
+    result_files()
+
+
+def result_files():
+    """4. neutral_fractions.dat as the reference writes it
+    (output_intensities through its MemoryMap), 2^3 subgrids of 4^3 cells,
+    an anisotropic off-centre box."""
+    ref = O.load_ref()
+    rng = np.random.default_rng(44)
+    import ctypes as C
+    import tempfile
+    box = np.array([-1.3 * PC, 0.2 * PC, -0.7 * PC, 2.1 * PC, 1.7 * PC, 2.6 * PC])
+    ncell = np.array([8, 8, 8], np.int32)
+    nsub = np.array([2, 2, 2], np.int32)
+    nc = int(np.prod(ncell))
+    n = 1e8 * np.exp(rng.normal(0, 1, nc))
+    xn = rng.uniform(0, 1, nc)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "neutral_fractions.dat")
+        ref.ref_output_dat.restype = C.c_size_t
+        size = ref.ref_output_dat(O.ptr(box, O._dp), O.ptr(ncell, O._ip),
+                                  O.ptr(nsub, O._ip), O.ptr(n, O._dp),
+                                  O.ptr(xn, O._dp), os.fsencode(path))
+        data = np.fromfile(path, np.uint8)
+    assert data.size == size
+    np.savez_compressed(os.path.join(HERE, "neutral_fractions_dat_8.npz"),
+                        box=box, ncell=ncell, nsub=nsub, n=n, xn=xn, dat=data)
+    print("result-file fixture written")
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["result-files"]:
+        result_files()
+    else:
+        main()
