@@ -113,6 +113,14 @@ const char *cmi_last_error(const cmi_ctx *ctx);
 /* Select the transport engine and its tuning (tuning may be NULL). */
 int cmi_set_engine(cmi_ctx *ctx, int engine, const cmi_tuning *tuning);
 
+/* Reemission (testDensitySubGrid_omp.cpp:675-702 do_reemission, the INSIDE
+ * buffer enabled at 1188-1191, reemission_probability of the parameter file):
+ * an absorbed packet is reemitted from its absorption point with this
+ * probability, with a new isotropic direction and target optical depth.
+ * 0 (the default) disables it, as in the reference.  Runs in cmi_transport on
+ * the tiles engine; the walk engine rejects a non-zero probability. */
+int cmi_set_reemission(cmi_ctx *ctx, double probability);
+
 /* Copy number density and neutral fraction in (either may be NULL to keep
  * the current values).  Replaces DensitySubGrid::set_number_density and
  * set_neutral_fraction (DensitySubGrid.hpp:1116-1131). */

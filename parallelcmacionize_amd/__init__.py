@@ -75,6 +75,7 @@ SIGNATURES = {
     "cmi_destroy": (None, [_vp]),
     "cmi_last_error": (C.c_char_p, [_vp]),
     "cmi_set_engine": (C.c_int, [_vp, C.c_int, C.POINTER(Tuning)]),
+    "cmi_set_reemission": (C.c_int, [_vp, C.c_double]),
     "cmi_set_fields": (C.c_int, [_vp, _dp, _dp]),
     "cmi_set_intensity": (C.c_int, [_vp, _dp]),
     "cmi_get_fields": (C.c_int, [_vp, _dp, _dp]),
@@ -222,6 +223,9 @@ class Context:
         t.walk_blocks = int(walk_blocks)
         t.lds_fp32 = int(lds_fp32)
         self._check(self.lib.cmi_set_engine(self.h, int(engine), C.byref(t)))
+
+    def set_reemission(self, probability):
+        self._check(self.lib.cmi_set_reemission(self.h, float(probability)))
 
     def set_fields(self, nH=None, xn=None):
         a = None if nH is None else np.ascontiguousarray(nH, dtype=np.float64)

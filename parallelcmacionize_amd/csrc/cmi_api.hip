@@ -584,6 +584,14 @@ int cmi_emit(cmi_ctx *c, uint64_t npackets, uint64_t first, uint64_t count,
   return CMI_OK;
 }
 
+int cmi_set_reemission(cmi_ctx *c, double probability) {
+  if (!c) return CMI_ERR_ARG;
+  if (!(probability >= 0. && probability < 1.))
+    return fail(c, CMI_ERR_ARG, "reemission probability must be in [0, 1)");
+  c->reemission = probability;
+  return CMI_OK;
+}
+
 int cmi_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
                   uint32_t iteration, double sigma, cmi_stats *stats) {
   if (!c) return CMI_ERR_ARG;
@@ -594,6 +602,8 @@ int cmi_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   if (rc != CMI_OK) return rc;
   if (c->engine == CMI_ENGINE_TILES)
     return tiles_transport(c, npackets, seed, iteration, sigma, stats);
+  if (c->reemission > 0.)
+    return fail(c, CMI_ERR_ARG, "reemission runs on the tiles engine");
   const int blocks = c->tuning.walk_blocks > 0 ? c->tuning.walk_blocks
                                                : c->num_cus * 8;
   CMI_TRY_HIP(c, hipMemsetAsync(c->d_cnt, 0, sizeof(Counters), c->stream));

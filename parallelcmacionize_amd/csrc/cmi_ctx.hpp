@@ -37,6 +37,7 @@ struct cmi_ctx {
   std::vector<double> h_partial;
   std::vector<double> src_pos, src_L;
   double sigma = 6.3e-22;  // cross section folded into kappa (FP32 mode)
+  double reemission = 0.;  // reemission probability (tiles engine)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double kernel_time_sum = 0.;
   uint64_t kernel_calls = 0;

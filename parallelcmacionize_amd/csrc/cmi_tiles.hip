@@ -60,7 +60,8 @@ struct TileGeo {
   int tile_cells;
   int ntiles;        // tiles of all problems
   int nprob;
-  int pad_;
+  float reemit_p;    // reemission probability of an absorbed packet (0: off)
+  uint32_t reemit_key[2];     // Philox key of the reemission draws
   const ProbGeo *probs;       // [nprob]
   const uint16_t *tile_prob;  // [ntiles] problem of each tile
 };
@@ -501,7 +502,13 @@ struct AccAtomic<double> {
   __device__ static void add(double *p, float v) { atomicAdd(p, (double)v); }
 };
 
-template <class Acc, int kThreads, bool kList>
+// kReemit: an absorbed packet is reemitted with probability G.reemit_p
+// (do_reemission, testDensitySubGrid_omp.cpp:675-702): a new isotropic
+// direction and target optical depth from the absorption point, in the same
+// cell.  The draws are Philox4x32-10 keyed by the transport's seed and
+// iteration, with the absorption point's coordinate bits, the tile and the
+// packet's reemission count as the counter.
+template <class Acc, int kThreads, bool kList, bool kReemit = false>
 __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     TileGeo G, const float *__restrict__ kappa, double *__restrict__ J,
     const Desc *__restrict__ desc, const uint32_t *__restrict__ ndesc_ptr,
@@ -576,7 +583,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
     float tm0 = 0.f, tm1 = 0.f, tm2 = 0.f, td0 = 0.f, td1 = 0.f, td2 = 0.f;
     float t = 0.f, tau = 0.f;
-    uint32_t my_pid = 0, vis_cross = 0;
+    uint32_t my_pid = 0, vis_cross = 0, n_reemit = 0;
 
     for (;;) {
       const uint64_t fin = __ballot(pending);
@@ -652,6 +659,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           l2 = gz - oz;
           ++my_hops;
           vis_cross = 0;
+          n_reemit = 0;
           steps = 0;
           t = 0.f;
           has_nxt = false;
@@ -708,6 +716,51 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         const float sa = kap > 0.f ? __fdividef(tau, kap) : 0.f;
         AccAtomic<Acc>::add(&accL[off], sa);
         t = t + sa;
+        if (kReemit) {
+          const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
+          const u32x4 r = philox4x32_10(
+              u32x4{__float_as_uint(q0), __float_as_uint(q1), __float_as_uint(q2),
+                    (uint32_t)tile ^ (n_reemit << 24)},
+              G.reemit_key[0], G.reemit_key[1]);
+          const float k24 = 1.f / 16777216.f;
+          if (((float)(r.x >> 8) + 0.5f) * k24 < G.reemit_p) {
+            // get_random_direction, testDensitySubGrid_omp.cpp:529-545, and
+            // tau = -ln u: FP32 draws on 24-bit uniforms in (0, 1)
+            const float cost = 2.f * (((float)(r.y >> 8) + 0.5f) * k24) - 1.f;
+            const float phi = 6.28318530717958647692f * (((float)(r.z >> 8) + 0.5f) * k24);
+            const float sint = sqrtf(fmaxf(1.f - cost * cost, 0.f));
+            float sp, cp;
+            sincosf(phi, &sp, &cp);
+            d0 = sint * cp * (float)P.unit_scale[0];
+            d1 = sint * sp * (float)P.unit_scale[1];
+            d2 = cost * (float)P.unit_scale[2];
+            tau = -__logf(((float)(r.w >> 8) + 0.5f) * k24);
+            p0 = q0;
+            p1 = q1;
+            p2 = q2;
+            t = 0.f;
+            ++n_reemit;
+            steps = 0;
+            const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
+                        i2 = __builtin_amdgcn_rcpf(d2);
+            tm0 = d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
+                           : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX);
+            tm1 = d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
+                           : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX);
+            tm2 = d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
+                           : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX);
+            tm0 = fmaxf(tm0, 0.f);
+            tm1 = fmaxf(tm1, 0.f);
+            tm2 = fmaxf(tm2, 0.f);
+            td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
+            td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
+            td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+            so0 = d0 > 0.f ? st0 : -st0;
+            so1 = d1 > 0.f ? st1 : -st1;
+            so2 = d2 > 0.f ? 1 : -1;
+            continue;
+          }
+        }
         live = false;
         pending = true;
         outcome = 0;
@@ -972,6 +1025,8 @@ static std::vector<ProbGeo> ctx_problems(const cmi_ctx *c) {
   return {P};
 }
 
+constexpr int kReemitRoundCap = 100000;
+
 static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
   TileGeo G{};
   for (int a = 0; a < 3; ++a) {
@@ -1110,11 +1165,25 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
                            unsigned long long *prob_cross) {
   TilesEngine *e = c->tiles;
 #define CMI_PROC(ACC, TH, LIST)                                                  \
-  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST>), dim3(R.grid), dim3(TH), \
+  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST, false>), dim3(R.grid), dim3(TH), \
                      R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc, \
                      e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,   \
                      LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po, \
                      c->d_cnt, prob_cross, R.refill)
+  if (G.reemit_p > 0.f) {
+    // reemission: FP64 LDS accumulators, emission-driven transports only
+    if (R.threads == 256)
+      hipLaunchKernelGGL((tiles_process_kernel<double, 256, false, true>), dim3(R.grid),
+                         dim3(256), R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,
+                         e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,
+                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill);
+    else
+      hipLaunchKernelGGL((tiles_process_kernel<double, 512, false, true>), dim3(R.grid),
+                         dim3(512), R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,
+                         e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,
+                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill);
+    return;
+  }
   if (with_pid) {
     if (R.fp32acc) {
       if (R.threads == 256) CMI_PROC(float, 256, true); else CMI_PROC(float, 512, true);
@@ -1151,14 +1220,17 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
   launch_plan(c, G, R, false);
   hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
                      0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
-  const int max_rounds = max_rounds_of(e->probs) + 2;
+  // straight rays visit a bounded number of tiles; reemitted packets turn,
+  // so their rounds are bounded only by the cap
+  const int max_rounds = G.reemit_p > 0.f ? kReemitRoundCap : max_rounds_of(e->probs) + 2;
+  const int timed = std::min<int>(max_rounds, (int)(e->ev.size() / 2));
   int r = 0;
   for (; r < max_rounds; ++r) {
     // round r reads P[0] (through idx) and writes its survivors to P[1];
     // the buffers swap roles before the next round
-    CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r], c->stream));
+    if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r], c->stream));
     launch_process(c, G, R, with_pid, po, nullptr);
-    CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r + 1], c->stream));
+    if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r + 1], c->stream));
     CMI_TRY_HIP(c, hipGetLastError());
     // total[0] = packets processed this round = slots written to P[1];
     // keep it in total[1] for the scatter, then plan the next round
@@ -1183,12 +1255,13 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
   if (r >= max_rounds) return fail(c, CMI_ERR_DIRECTION, "tiles engine: round limit reached");
   // device time of every process launch (the dominant kernel)
   CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
-  for (int k = 0; k < r; ++k) {
+  const int nt = std::min(r, timed);
+  for (int k = 0; k < nt; ++k) {
     float ms = 0.f;
     CMI_TRY_HIP(c, hipEventElapsedTime(&ms, e->ev[2 * k], e->ev[2 * k + 1]));
     c->kernel_time_sum += ms * 1e-3;
   }
-  c->kernel_calls += (uint64_t)r;
+  c->kernel_calls += (uint64_t)nt;
   return CMI_OK;
 }
 
@@ -1233,7 +1306,10 @@ int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   int rc = ensure_engine(c, ctx_problems(c), (size_t)npackets, false);
   if (rc != CMI_OK) return rc;
   TilesEngine *e = c->tiles;
-  const TileGeo G = make_geo(c, e);
+  TileGeo G = make_geo(c, e);
+  G.reemit_p = (float)c->reemission;
+  G.reemit_key[0] = (uint32_t)seed ^ 0xA511E9B3u;
+  G.reemit_key[1] = (uint32_t)(seed >> 32) ^ (iteration * 0x9E3779B9u) ^ 0x63D83595u;
   fill_set_sources_single(c, e, seed, iteration);
   CMI_TRY_HIP(c, hipMemcpyAsync(e->d_src, e->h_src, sizeof(SetSources),
                                 hipMemcpyHostToDevice, c->stream));
