@@ -148,6 +148,19 @@ int cmi_interact_batch(cmi_ctx *ctx, uint32_t subgrid, int entry_dir,
                        cmi_photon *p, size_t n, int32_t *exit_dir,
                        uint32_t *crossings);
 
+/* DensitySubGrid::propagate (DensitySubGrid.hpp:1333-1472): the same walk
+ * without deposits into J.  Same arguments as cmi_interact_batch. */
+int cmi_propagate_batch(cmi_ctx *ctx, uint32_t subgrid, int entry_dir,
+                        cmi_photon *p, size_t n, int32_t *exit_dir,
+                        uint32_t *crossings);
+
+/* DensitySubGrid::compute_optical_depth (DensitySubGrid.hpp:1482-1604): walk
+ * to the subgrid's edge without deposits and without stopping; the optical
+ * depth of the path is added to the packet's tau. */
+int cmi_optical_depth_batch(cmi_ctx *ctx, uint32_t subgrid, int entry_dir,
+                            cmi_photon *p, size_t n, int32_t *exit_dir,
+                            uint32_t *crossings);
+
 /* Whole-grid transport of an explicit packet list: each packet starts INSIDE
  * start_sub[i] and is routed from subgrid to subgrid (the loop of
  * testDensitySubGrid_omp.cpp:607-666 and 1149-1329 with

@@ -176,6 +176,16 @@ static inline double std_min(double a, double b) { return (b < a) ? b : a; }
  * position). */
 int orc_interact(const orc_subgrid *s, orc_photon *photon, int input_direction,
                  uint32_t *crossings) {
+  return orc_walk(s, photon, input_direction, ORC_WALK_INTERACT, crossings);
+}
+
+/* The reference's three walk variants share the loop above:
+ * ORC_WALK_INTERACT (interact, 1180-1323), ORC_WALK_PROPAGATE (propagate,
+ * 1333-1472: no deposit) and ORC_WALK_OPTICAL_DEPTH (compute_optical_depth,
+ * 1482-1604: no deposit, the loop runs while the index is inside, and the
+ * path's optical depth is added to the packet's tau, 1592-1593). */
+int orc_walk(const orc_subgrid *s, orc_photon *photon, int input_direction,
+             int kind, uint32_t *crossings) {
   const double *direction = photon->dir;
   const double inverse_direction[3] = {1. / direction[0], 1. / direction[1],
                                        1. / direction[2]};
@@ -190,7 +200,8 @@ int orc_interact(const orc_subgrid *s, orc_photon *photon, int input_direction,
   int active_cell = orc_get_start_index(s, position, input_direction,
                                         three_index);
   uint32_t ncross = 0;
-  while (tau_done < tau_target && three_index[0] < s->ncell[0] &&
+  const int od = kind == ORC_WALK_OPTICAL_DEPTH;
+  while ((od || tau_done < tau_target) && three_index[0] < s->ncell[0] &&
          three_index[0] >= 0 && three_index[1] < s->ncell[1] &&
          three_index[1] >= 0 && three_index[2] < s->ncell[2] &&
          three_index[2] >= 0) {
@@ -215,7 +226,7 @@ int orc_interact(const orc_subgrid *s, orc_photon *photon, int input_direction,
     const double tau =
         lminsigma * s->n[active_cell] * s->xn[active_cell];
     tau_done += tau;
-    if (tau_done >= tau_target) {
+    if (!od && tau_done >= tau_target) {
       const double correction = (tau_done - tau_target) / tau;
       lmin *= (1. - correction);
       lminsigma = lmin * cross_section;
@@ -226,7 +237,7 @@ int orc_interact(const orc_subgrid *s, orc_photon *photon, int input_direction,
         }
       }
     }
-    s->J[active_cell] += lminsigma * photon_weight;
+    if (kind == ORC_WALK_INTERACT) s->J[active_cell] += lminsigma * photon_weight;
     for (int idim = 0; idim < 3; ++idim) {
       position[idim] =
           (l[idim] == lmin)
@@ -237,10 +248,10 @@ int orc_interact(const orc_subgrid *s, orc_photon *photon, int input_direction,
                   three_index[2];
     ++ncross;
   }
-  photon->tau = tau_target - tau_done;
+  photon->tau = od ? tau_target + tau_done : tau_target - tau_done;
   for (int a = 0; a < 3; ++a) photon->pos[a] = position[a] + s->anchor[a];
   if (crossings) *crossings += ncross;
-  if (tau_done >= tau_target) return ORC_INSIDE;
+  if (!od && tau_done >= tau_target) return ORC_INSIDE;
   return orc_get_output_direction(s, three_index);
 }
 

@@ -89,6 +89,9 @@ void orc_subgrid_init(orc_subgrid *s, const double box[6], const int ncell[3]);
 /* ---- traversal (DensitySubGrid.hpp:1180-1323) ---- */
 int orc_interact(const orc_subgrid *s, orc_photon *photon, int input_direction,
                  uint32_t *crossings);
+enum { ORC_WALK_INTERACT = 0, ORC_WALK_PROPAGATE = 1, ORC_WALK_OPTICAL_DEPTH = 2 };
+int orc_walk(const orc_subgrid *s, orc_photon *photon, int input_direction,
+             int kind, uint32_t *crossings);
 
 /* ---- grid of subgrids (testDensitySubGrid_omp.cpp:1805-1877) ---- */
 int orc_grid_create(orc_grid *g, const double box[6], const int ncell[3],

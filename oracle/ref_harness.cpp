@@ -209,6 +209,40 @@ void ref_interact_list(const double box[6], const int ncell[3],
             g.get_intensity_integral(ix, iy, iz);
 }
 
+/* DensitySubGrid::propagate (kind 1) or compute_optical_depth (kind 2) for
+ * a packet list.  Crossings from a shadow interact of the same packet (NaN
+ * weight): propagate walks exactly as interact; compute_optical_depth walks
+ * to the edge, as interact does with an infinite target optical depth. */
+extern "C" void ref_walk_list(const double box[6], const int ncell[3],
+                              const double *n, const double *xn, double *photons,
+                              size_t np, int input_direction, int kind,
+                              int32_t *exit_dir, uint32_t *crossings) {
+  DensitySubGrid g(box, ncell), shadow(box, ncell);
+  const size_t nc = (size_t)ncell[0] * ncell[1] * ncell[2];
+  for (size_t i = 0; i < nc; ++i) {
+    g.set_number_density(i, n[i]);
+    g.set_neutral_fraction(i, xn[i]);
+    g.set_intensity_integral(i, 0.);
+    shadow.set_number_density(i, n[i]);
+    shadow.set_neutral_fraction(i, xn[i]);
+    shadow.set_intensity_integral(i, 0.);
+  }
+  int nc3[3] = {ncell[0], ncell[1], ncell[2]};
+  HPhoton *hp = reinterpret_cast<HPhoton *>(photons);
+  for (size_t k = 0; k < np; ++k) {
+    Photon p, q;
+    to_photon(hp[k], p);
+    to_photon(hp[k], q);
+    q.set_weight(NAN);
+    if (kind == 2) q.set_target_optical_depth(INFINITY);
+    exit_dir[k] = kind == 2 ? g.compute_optical_depth(p, input_direction)
+                            : g.propagate(p, input_direction);
+    shadow.interact(q, input_direction);
+    crossings[k] = count_and_clear_nan(shadow, nc3);
+    from_photon(p, hp[k]);
+  }
+}
+
 /* Whole-grid transport of an explicit packet list, one packet at a time,
  * with the routing of testDensitySubGrid_omp.cpp:607-666 / 1149-1329
  * (INSIDE = absorbed, NEIGHBOUR_OUTSIDE = escaped, otherwise enter the

@@ -82,6 +82,10 @@ SIGNATURES = {
     "cmi_reset_intensity": (C.c_int, [_vp]),
     "cmi_interact_batch": (C.c_int, [_vp, C.c_uint32, C.c_int, _vp, C.c_size_t,
                                       _i32p, _u32p]),
+    "cmi_propagate_batch": (C.c_int, [_vp, C.c_uint32, C.c_int, _vp, C.c_size_t,
+                                      _i32p, _u32p]),
+    "cmi_optical_depth_batch": (C.c_int, [_vp, C.c_uint32, C.c_int, _vp, C.c_size_t,
+                                      _i32p, _u32p]),
     "cmi_transport_packets": (C.c_int, [_vp, _vp, _u32p, C.c_size_t, _i32p,
                                          _u32p, _u32p, C.POINTER(Stats)]),
     "cmi_set_sources": (C.c_int, [_vp, _dp, _dp, C.c_int]),
@@ -252,14 +256,20 @@ class Context:
         self._check(self.lib.cmi_get_fields(self.h, _p(J, _dp), _p(x, _dp)))
         return J, x
 
-    def interact_batch(self, subgrid, entry_dir, photons):
-        p = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE)
+    def interact_batch(self, subgrid, entry_dir, photons, kind="interact"):
+        """One subgrid, explicit packets: kind 'interact' (deposits),
+        'propagate' (no deposits) or 'optical_depth' (no stop, tau += path
+        optical depth).  Returns the updated packets, exit directions and
+        crossings."""
+        fn = {"interact": self.lib.cmi_interact_batch,
+              "propagate": self.lib.cmi_propagate_batch,
+              "optical_depth": self.lib.cmi_optical_depth_batch}[kind]
+        p = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE).copy()
         n = len(p)
         ex = np.zeros(n, np.int32)
         cr = np.zeros(n, np.uint32)
-        self._check(self.lib.cmi_interact_batch(self.h, int(subgrid),
-                                                int(entry_dir), p.ctypes.data,
-                                                n, _p(ex, _i32p), _p(cr, _u32p)))
+        self._check(fn(self.h, int(subgrid), int(entry_dir), p.ctypes.data, n,
+                       _p(ex, _i32p), _p(cr, _u32p)))
         return p, ex, cr
 
     def transport_packets(self, photons, start_sub):

@@ -431,12 +431,15 @@ int cmi_reset_intensity(cmi_ctx *c) {
   return CMI_OK;
 }
 
-int cmi_interact_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir,
-                       cmi_photon *p, size_t n, int32_t *exit_dir,
-                       uint32_t *crossings) {
+}  // extern "C"
+
+namespace {
+// one subgrid, explicit packets, one of the reference's three walk variants
+int walk_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir, cmi_photon *p, size_t n,
+               int32_t *exit_dir, uint32_t *crossings, int kind) {
   if (!c) return CMI_ERR_ARG;
   if (c->precision != CMI_PREC_FP64)
-    return fail(c, CMI_ERR_ARG, "cmi_interact_batch needs an FP64 context");
+    return fail(c, CMI_ERR_ARG, "per-subgrid walks need an FP64 context");
   if (subgrid >= c->nsubgrid || entry_dir < 0 || entry_dir >= 27 ||
       (n && (!p || !exit_dir || !crossings)))
     return fail(c, CMI_ERR_ARG, "bad subgrid, entry direction or buffers");
@@ -452,7 +455,7 @@ int cmi_interact_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir,
                                           hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess)
     e = launch_interact_batch(c->g, c->f.n, c->f.xn, c->f.J, (int)subgrid,
-                              entry_dir, dp, n, de, dc, c->stream);
+                              entry_dir, dp, n, de, dc, c->stream, kind);
   if (e == hipSuccess) e = hipMemcpyAsync(p, dp, n * sizeof(PhotonRec),
                                           hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(exit_dir, de, n * sizeof(int32_t),
@@ -469,6 +472,24 @@ int cmi_interact_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir,
     if (exit_dir[i] < 0)
       return fail(c, CMI_ERR_DIRECTION, "a packet hit the walk bound");
   return CMI_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cmi_interact_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir, cmi_photon *p,
+                       size_t n, int32_t *exit_dir, uint32_t *crossings) {
+  return walk_batch(c, subgrid, entry_dir, p, n, exit_dir, crossings, 0);
+}
+
+int cmi_propagate_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir, cmi_photon *p,
+                        size_t n, int32_t *exit_dir, uint32_t *crossings) {
+  return walk_batch(c, subgrid, entry_dir, p, n, exit_dir, crossings, 1);
+}
+
+int cmi_optical_depth_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir, cmi_photon *p,
+                            size_t n, int32_t *exit_dir, uint32_t *crossings) {
+  return walk_batch(c, subgrid, entry_dir, p, n, exit_dir, crossings, 2);
 }
 
 int cmi_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub,

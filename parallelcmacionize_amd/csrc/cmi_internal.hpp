@@ -30,7 +30,7 @@ hipError_t launch_interact_batch(const GridDev &g, const double *n,
                                  const double *xn, double *J, int sub,
                                  int in_dir, PhotonRec *p, size_t np,
                                  int32_t *exit_dir, uint32_t *crossings,
-                                 hipStream_t st);
+                                 hipStream_t st, int kind = 0);
 hipError_t launch_walk_list(const GridDev &g, const FieldsDev &f, int prec,
                             PhotonRec *p, const uint32_t *start_sub, size_t np,
                             int32_t *status, uint32_t *crossings,

@@ -73,7 +73,15 @@ __device__ __forceinline__ void start_index(Walker<T> &w, int in_dir,
 }
 
 // ---- Mode64: one loop iteration of DensitySubGrid::interact --------------
-struct Mode64 {
+// kKind selects the reference's walk variant: kInteract
+// (DensitySubGrid::interact, 1180-1323: deposit, stop at the target optical
+// depth), kPropagate (propagate, 1333-1472: the same walk without deposits)
+// or kOpticalDepth (compute_optical_depth, 1482-1604: no deposits, no stop,
+// the optical depth of the whole path accumulated).
+enum WalkKind { kInteract = 0, kPropagate = 1, kOpticalDepth = 2 };
+
+template <int kKind>
+struct Mode64T {
   using real = double;
   const double *n;
   const double *xn;
@@ -96,7 +104,7 @@ struct Mode64 {
     double lminsigma = lmin * sigma;
     const double tau = lminsigma * n[c] * xn[c];
     w.tau_done += tau;
-    const bool absorbed = w.tau_done >= w.tau_target;
+    const bool absorbed = kKind != kOpticalDepth && w.tau_done >= w.tau_target;
     if (absorbed) {
       const double correction = (w.tau_done - w.tau_target) / tau;
       lmin *= (1. - correction);
@@ -107,12 +115,13 @@ struct Mode64 {
         if (l[a] == lmin) w.idx[a] += (w.dir[a] > 0.) ? 1 : -1;
       }
     }
-    atomicAdd(&J[c], lminsigma * weight);
+    if (kKind == kInteract) atomicAdd(&J[c], lminsigma * weight);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       w.pos[a] = (l[a] == lmin) ? ((w.dir[a] > 0.) ? hi[a] : lo[a])
                                 : w.pos[a] + lmin * w.dir[a];
     }
+    if (kKind == kOpticalDepth) return inside3(w.idx, g.sc) ? kContinue : kLeft;
     if (absorbed) return kAbsorbed;
     // a NaN optical depth ends the reference loop too (its condition
     // tau_done < tau_target fails); the packet then counts as absorbed when
@@ -122,6 +131,8 @@ struct Mode64 {
     return inside3(w.idx, g.sc) ? kContinue : kLeft;
   }
 };
+
+using Mode64 = Mode64T<kInteract>;
 
 // ---- Mode32: the same step in FP32 cell units ----------------------------
 struct Mode32 {
@@ -286,6 +297,7 @@ __device__ __forceinline__ void wave_add(unsigned long long *dst,
 // Kernel: DensitySubGrid::interact on one subgrid for an explicit packet list
 // (the deterministic oracle's entry point).  One thread per packet.
 // ---------------------------------------------------------------------------
+template <int kKind>
 __global__ void interact_batch_kernel(GridDev g, const double *n,
                                       const double *xn, double *J, int sub,
                                       int in_dir, PhotonRec *p, size_t np,
@@ -297,9 +309,10 @@ __global__ void interact_batch_kernel(GridDev g, const double *n,
   PhotonRec ph = p[i];
   Walker<double> w;
   load_packet(w, g, sub, ph.pos, ph.dir, ph.tau, in_dir);
-  Mode64 m{n, xn, J};
+  Mode64T<kKind> m{n, xn, J};
   uint32_t nc = 0;
-  int r = visit_open(w, g);
+  int r = kKind == kOpticalDepth ? (inside3(w.idx, g.sc) ? kContinue : kLeft)
+                                 : visit_open(w, g);
   while (r == kContinue && (int)nc < max_steps) {
     r = m.step(w, g, ph.sigma, ph.weight);
     ++nc;
@@ -313,7 +326,9 @@ __global__ void interact_batch_kernel(GridDev g, const double *n,
     out = exit_direction(w.idx[0], w.idx[1], w.idx[2], g.sc[0], g.sc[1],
                          g.sc[2]);
   }
-  ph.tau = w.tau_target - w.tau_done;
+  // interact / propagate: the remaining optical depth (DensitySubGrid.hpp:
+  // 1307, 1456); compute_optical_depth: the accumulated one (1592-1593)
+  ph.tau = kKind == kOpticalDepth ? w.tau_target + w.tau_done : w.tau_target - w.tau_done;
   abs_position(w, g, ph.pos);
   p[i] = ph;
   exit_dir[i] = out;
@@ -495,13 +510,19 @@ hipError_t launch_interact_batch(const GridDev &g, const double *n,
                                  const double *xn, double *J, int sub,
                                  int in_dir, PhotonRec *p, size_t np,
                                  int32_t *exit_dir, uint32_t *crossings,
-                                 hipStream_t st) {
+                                 hipStream_t st, int kind) {
   if (np == 0) return hipSuccess;
   const int max_steps = g.sc[0] + g.sc[1] + g.sc[2] + 8;
   const unsigned blocks = (unsigned)((np + 255) / 256);
-  hipLaunchKernelGGL(interact_batch_kernel, dim3(blocks), dim3(256), 0, st, g,
-                     n, xn, J, sub, in_dir, p, np, exit_dir, crossings,
-                     max_steps);
+  if (kind == kPropagate)
+    hipLaunchKernelGGL(interact_batch_kernel<kPropagate>, dim3(blocks), dim3(256), 0, st, g,
+                       n, xn, J, sub, in_dir, p, np, exit_dir, crossings, max_steps);
+  else if (kind == kOpticalDepth)
+    hipLaunchKernelGGL(interact_batch_kernel<kOpticalDepth>, dim3(blocks), dim3(256), 0, st,
+                       g, n, xn, J, sub, in_dir, p, np, exit_dir, crossings, max_steps);
+  else
+    hipLaunchKernelGGL(interact_batch_kernel<kInteract>, dim3(blocks), dim3(256), 0, st, g,
+                       n, xn, J, sub, in_dir, p, np, exit_dir, crossings, max_steps);
   return hipGetLastError();
 }
 

@@ -137,6 +137,33 @@ def main():
                         box=box, ncell=ncell, n=n, J=J, L=L, N=N, xn=xo)
     print("fixtures written to", HERE)
     result_files()
+    walk_variants()
+
+
+def walk_variants():
+    """5. DensitySubGrid::propagate and compute_optical_depth on one
+    anisotropic 12x16x8 subgrid, clumpy field, three entry sides."""
+    ref = O.load_ref()
+    rng = np.random.default_rng(55)
+    box = np.array([1.0 * PC, -2.0 * PC, 0.5 * PC, 1.5 * PC, 2.0 * PC, 1.0 * PC])
+    ncell = np.array([12, 16, 8], np.int32)
+    nc = int(np.prod(ncell))
+    n = 1e8 * np.exp(rng.normal(0, 1, nc))
+    xn = 10 ** rng.uniform(-6, -2, nc)
+    out = {"box": box, "ncell": ncell, "n": n, "xn": xn}
+    for k, in_dir in enumerate([0, 24, 7]):  # INSIDE, FACE_Y_N, CORNER_NNP
+        p = entry_packets(rng, 300, box, ncell, in_dir, 0.5)
+        if in_dir == 0:
+            p["pos"] = box[:3] + rng.uniform(0, 1, (300, 3)) * box[3:]
+        out["in_dir_%d" % k] = np.int32(in_dir)
+        out["photons_in_%d" % k] = p.copy()
+        for kind, name in ((1, "propagate"), (2, "optical_depth")):
+            q, ex, cr = O.ref_walk_list(ref, box, ncell, n, xn, p, in_dir, kind)
+            out["%s_out_%d" % (name, k)] = q
+            out["%s_exit_%d" % (name, k)] = ex
+            out["%s_crossings_%d" % (name, k)] = cr
+    np.savez_compressed(os.path.join(HERE, "walk_variants_12.npz"), **out)
+    print("walk-variant fixture written")
 
 
 def result_files():
@@ -169,5 +196,7 @@ def result_files():
 if __name__ == "__main__":
     if sys.argv[1:] == ["result-files"]:
         result_files()
+    elif sys.argv[1:] == ["walk-variants"]:
+        walk_variants()
     else:
         main()

@@ -56,6 +56,8 @@ def load_oracle():
     lib.orc_subgrid_init.argtypes = [C.POINTER(OrcSubgrid), _dp, _ip]
     lib.orc_interact.argtypes = [C.POINTER(OrcSubgrid), C.c_void_p, C.c_int,
                                  _u32p]
+    lib.orc_walk.argtypes = [C.POINTER(OrcSubgrid), C.c_void_p, C.c_int,
+                             C.c_int, _u32p]
     lib.orc_is_compatible_output_direction.argtypes = [_dp, C.c_int]
     lib.orc_is_compatible_input_direction.argtypes = [_dp, C.c_int]
     lib.orc_grid_create.argtypes = [C.POINTER(OrcGrid), _dp, _ip, _ip]
@@ -97,6 +99,8 @@ def load_ref(fast=False):
     lib.ref_get_start_index.argtypes = [_dp, _ip, _dp, C.c_int, _ip]
     lib.ref_interact_list.argtypes = [_dp, _ip, _dp, _dp, _dp, C.c_void_p,
                                       C.c_size_t, C.c_int, _i32p, _u32p]
+    lib.ref_walk_list.argtypes = [_dp, _ip, _dp, _dp, C.c_void_p, C.c_size_t,
+                                  C.c_int, C.c_int, _i32p, _u32p]
     lib.ref_transport_list.argtypes = [_dp, _ip, _ip, _dp, _dp, _dp,
                                        C.c_void_p, _u32p, C.c_size_t, _i32p,
                                        _u32p, _u32p]
@@ -220,3 +224,19 @@ def ref_interact_list(ref, box, ncell, n, xn, photons, input_direction):
                           ptr(xx, _dp), ptr(J, _dp), photons.ctypes.data, m,
                           input_direction, ptr(ex, _i32p), ptr(cr, _u32p))
     return J, ex, cr
+
+
+def ref_walk_list(ref, box, ncell, n, xn, photons, input_direction, kind):
+    """The reference's propagate (kind 1) / compute_optical_depth (kind 2)."""
+    b = np.ascontiguousarray(box, dtype=np.float64)
+    nc = np.ascontiguousarray(ncell, dtype=np.int32)
+    nn = np.ascontiguousarray(n, dtype=np.float64)
+    xx = np.ascontiguousarray(xn, dtype=np.float64)
+    p = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE).copy()
+    m = len(p)
+    ex = np.zeros(m, np.int32)
+    cr = np.zeros(m, np.uint32)
+    ref.ref_walk_list(ptr(b, _dp), ptr(nc, _ip), ptr(nn, _dp), ptr(xx, _dp),
+                      p.ctypes.data, m, input_direction, kind, ptr(ex, _i32p),
+                      ptr(cr, _u32p))
+    return p, ex, cr

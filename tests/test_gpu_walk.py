@@ -234,3 +234,22 @@ def test_stromgren_radius(prec):
     Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)
     print("R/Rs", R / Rs)
     assert abs(R / Rs - 1) < 0.02
+
+
+@pytest.mark.parametrize("kind", ["propagate", "optical_depth"])
+def test_walk_variants_match_golden(golden, kind):
+    """DensitySubGrid::propagate / compute_optical_depth: packets, exits and
+    crossings bit-exact with the reference's output; J untouched."""
+    g = golden("walk_variants_12.npz")
+    with cmi.Context(g["box"], g["ncell"], [1, 1, 1], cmi.PREC_FP64) as ctx:
+        ctx.set_fields(g["n"], g["xn"])
+        ctx.reset_intensity()
+        for k in range(3):
+            print(kind, "golden case", k, flush=True)
+            p, ex, cr = ctx.interact_batch(0, int(g["in_dir_%d" % k]),
+                                           g["photons_in_%d" % k], kind=kind)
+            assert (ex == g["%s_exit_%d" % (kind, k)]).all()
+            assert (cr == g["%s_crossings_%d" % (kind, k)]).all()
+            assert _bitwise_equal(p, g["%s_out_%d" % (kind, k)])
+        J, _ = ctx.get_fields()
+        assert (J == 0).all()

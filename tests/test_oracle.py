@@ -456,3 +456,114 @@ def test_source_counts(oracle):
     assert out.sum() == 1000003
     base = np.floor(L / L.sum() * 1000003)
     assert ((out - base) >= 0).all() and ((out - base) <= 1).all()
+
+
+# ---- the reference's other two walk variants -------------------------------
+
+def _walk_oracle(oracle, box, ncell, n, xn, photons, in_dir, kind):
+    s = O.make_subgrid(oracle, box, ncell)
+    nn = np.ascontiguousarray(n, dtype=np.float64).copy()
+    xx = np.ascontiguousarray(xn, dtype=np.float64).copy()
+    J = np.zeros_like(nn)
+    s.n = O.ptr(nn, O._dp)
+    s.xn = O.ptr(xx, O._dp)
+    s.J = O.ptr(J, O._dp)
+    p = photons.copy()
+    ex = np.zeros(len(p), np.int32)
+    cr = np.zeros(len(p), np.uint32)
+    one = np.zeros(1, np.uint32)
+    for i in range(len(p)):
+        one[0] = 0
+        ex[i] = oracle.orc_walk(C.byref(s), p[i:i + 1].ctypes.data, in_dir, kind,
+                                O.ptr(one, O._u32p))
+        cr[i] = one[0]
+    return p, ex, cr, J
+
+
+@pytest.mark.parametrize("kind,name", [(1, "propagate"), (2, "optical_depth")])
+def test_walk_variants_match_golden(oracle, golden, kind, name):
+    g = golden("walk_variants_12.npz")
+    for k in range(3):
+        p, ex, cr, J = _walk_oracle(oracle, g["box"], g["ncell"], g["n"], g["xn"],
+                                    g["photons_in_%d" % k], int(g["in_dir_%d" % k]),
+                                    kind)
+        assert (ex == g["%s_exit_%d" % (name, k)]).all()
+        assert (cr == g["%s_crossings_%d" % (name, k)]).all()
+        assert (p.view(np.uint8) == g["%s_out_%d" % (name, k)].view(np.uint8)).all()
+        assert (J == 0).all()   # neither variant deposits
+
+
+def test_walk_variants_against_live_reference(oracle, ref):
+    rng = np.random.default_rng(12)
+    box = np.array([0.3 * PC, 0.1 * PC, -0.5 * PC, 0.9 * PC, 1.4 * PC, 0.7 * PC])
+    ncell = np.array([10, 14, 6], np.int32)
+    nc = int(np.prod(ncell))
+    n = 1e8 * np.exp(rng.normal(0, 1.5, nc))
+    xn = 10 ** rng.uniform(-7, -1, nc)
+    for in_dir in (0, 5, 17, 21):
+        m = 250
+        p = np.zeros(m, O.PHOTON_DTYPE)
+        p["pos"] = box[:3] + rng.uniform(0, 1, (m, 3)) * box[3:]
+        d = rng.normal(size=(m, 3))
+        p["dir"] = d / np.linalg.norm(d, axis=1)[:, None]
+        off = np.zeros(3, np.int32)
+        oracle.orc_direction_offset(in_dir, O.ptr(off, O._ip))
+        for a in range(3):
+            if off[a]:
+                p["pos"][:, a] = box[a] + (box[3 + a] if off[a] > 0 else 0.)
+                p["dir"][:, a] = -off[a] * np.abs(p["dir"][:, a])
+        p["tau"] = -np.log(rng.uniform(0, 1, m))
+        p["sigma"] = 6.3e-22
+        p["weight"] = 1.0
+        for kind in (1, 2):
+            po, ex, cr, _ = _walk_oracle(oracle, box, ncell, n, xn, p, in_dir, kind)
+            q, exr, crr = O.ref_walk_list(ref, box, ncell, n, xn, p, in_dir, kind)
+            assert (ex == exr).all() and (cr == crr).all()
+            assert (po.view(np.uint8) == q.view(np.uint8)).all()
+
+
+def test_propagate_is_interact_without_deposits(oracle):
+    """Same packet states and exits as interact, J untouched."""
+    rng = np.random.default_rng(13)
+    box = np.array([0.0, 0.0, 0.0, PC, PC, PC])
+    ncell = np.array([8, 8, 8], np.int32)
+    n = 1e8 * np.exp(rng.normal(0, 1, 512))
+    xn = 10 ** rng.uniform(-5, -1, 512)
+    p = np.zeros(400, O.PHOTON_DTYPE)
+    p["pos"] = rng.uniform(0, PC, (400, 3))
+    d = rng.normal(size=(400, 3))
+    p["dir"] = d / np.linalg.norm(d, axis=1)[:, None]
+    p["tau"] = -np.log(rng.uniform(0, 1, 400))
+    p["sigma"] = 6.3e-22
+    p["weight"] = 1.0
+    a, exa, cra, Ja = _interact_oracle(oracle, box, ncell, n, xn, p, 0)
+    b, exb, crb, Jb = _walk_oracle(oracle, box, ncell, n, xn, p, 0, 1)
+    assert (a.view(np.uint8) == b.view(np.uint8)).all()
+    assert (exa == exb).all() and (cra == crb).all()
+    assert Ja.sum() > 0 and (Jb == 0).all()
+
+
+def test_optical_depth_of_uniform_box_is_chord(oracle):
+    """Uniform n x: the accumulated optical depth is sigma n x times the
+    chord from the start point to the box edge, whatever the target."""
+    rng = np.random.default_rng(14)
+    L = 2.0 * PC
+    box = np.array([0.0, 0.0, 0.0, L, L, L])
+    ncell = np.array([16, 16, 16], np.int32)
+    nval, xval, sig = 1e8, 3e-3, 6.3e-22
+    m = 300
+    p = np.zeros(m, O.PHOTON_DTYPE)
+    p["pos"] = rng.uniform(0.05 * L, 0.95 * L, (m, 3))
+    d = rng.normal(size=(m, 3))
+    p["dir"] = d / np.linalg.norm(d, axis=1)[:, None]
+    p["tau"] = rng.uniform(0, 2, m)
+    p["sigma"] = sig
+    p["weight"] = 1.0
+    q, ex, cr, J = _walk_oracle(oracle, box, ncell, np.full(4096, nval),
+                                np.full(4096, xval), p, 0, 2)
+    t = np.where(p["dir"] > 0, (L - p["pos"]) / p["dir"], -p["pos"] / p["dir"])
+    chord = t.min(axis=1)
+    assert np.allclose(q["tau"] - p["tau"], sig * nval * xval * chord, rtol=1e-9)
+    assert (ex != 0).all()
+    end = p["pos"] + chord[:, None] * p["dir"]
+    assert np.allclose(q["pos"], end, rtol=0, atol=1e-9 * L)
