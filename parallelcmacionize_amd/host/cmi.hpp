@@ -193,6 +193,12 @@ class DensitySubGrid {
   // direction is written to exit_dirs (size() entries).
   inline void interact(PhotonBuffer &buffer, int *exit_dirs,
                        uint32_t *crossings = nullptr);
+  // DensitySubGrid::propagate (DensitySubGrid.hpp:1333): the walk without
+  // deposits.
+  inline int propagate(Photon &photon, int input_direction);
+  // DensitySubGrid::compute_optical_depth (DensitySubGrid.hpp:1482): walk to
+  // the subgrid's edge, adding the path's optical depth to the packet's.
+  inline int compute_optical_depth(Photon &photon, int input_direction);
   inline unsigned int get_neighbour(int direction) const;
   // DensitySubGrid.hpp:483-506 (public in the reference for unit tests)
   inline int get_output_direction(const int three_index[3]) const;
@@ -274,6 +280,22 @@ class DensitySubGridGrid {
     return l1;
   }
 
+  // reemission_probability of the reference's parameter file (its
+  // do_reemission, testDensitySubGrid_omp.cpp:675-702); tiles engine
+  void set_reemission_probability(double p) {
+    check(cmi_set_reemission(ctx_, p), "cmi_set_reemission");
+  }
+  // output_neutral_fractions (testDensitySubGrid_omp.cpp:394-440) and
+  // output_neutral_fractions_binary (507-521): the reference's result files
+  void output_neutral_fractions(const char *dat_path = "neutral_fractions.dat") {
+    sync_in();
+    check(cmi_write_results(ctx_, dat_path, nullptr), "cmi_write_results");
+  }
+  void output_neutral_fractions_binary(const char *bin_path = "neutral_fractions.bin") {
+    sync_in();
+    check(cmi_write_results(ctx_, nullptr, bin_path), "cmi_write_results");
+  }
+
   // --- used by the DensitySubGrid views ---
   void sync_in() {
     if (dirty_fields_) {
@@ -332,6 +354,28 @@ inline void DensitySubGrid::interact(PhotonBuffer &buffer, int *exit_dirs,
             "cmi_interact_batch");
   if (crossings) std::memcpy(crossings, nc.data(), nc.size() * sizeof(uint32_t));
   g_->host_stale_ = true;
+}
+
+inline int DensitySubGrid::propagate(Photon &photon, int input_direction) {
+  g_->sync_out();
+  g_->sync_in();
+  int32_t out = 0;
+  uint32_t nc = 0;
+  g_->check(cmi_propagate_batch(g_->ctx_, index_, input_direction, &photon.raw(), 1,
+                                &out, &nc),
+            "cmi_propagate_batch");
+  return out;
+}
+
+inline int DensitySubGrid::compute_optical_depth(Photon &photon, int input_direction) {
+  g_->sync_out();
+  g_->sync_in();
+  int32_t out = 0;
+  uint32_t nc = 0;
+  g_->check(cmi_optical_depth_batch(g_->ctx_, index_, input_direction, &photon.raw(), 1,
+                                    &out, &nc),
+            "cmi_optical_depth_batch");
+  return out;
 }
 
 inline unsigned int DensitySubGrid::get_neighbour(int direction) const {

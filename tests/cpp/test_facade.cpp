@@ -134,6 +134,27 @@ int main() {
       CHECK(single[k].get_target_optical_depth() == buf[k].get_target_optical_depth(),
             "tau of packet %u", k);
     }
+    // ---- propagate walks like interact without deposits; the optical
+    //      depth walk reaches the edge ----
+    {
+      for (unsigned int k = 0; k < 20; ++k) {
+        Photon a = buf[k], b = buf[k];
+        a.set_position(0.1 * pc, 0.2 * pc, 0.3 * pc);
+        b = a;
+        a.set_target_optical_depth(0.05);
+        b.set_target_optical_depth(0.05);
+        const int ea = grid[7].interact(a, TRAVELDIRECTION_INSIDE);
+        const int eb = grid[7].propagate(b, TRAVELDIRECTION_INSIDE);
+        CHECK(ea == eb && a.get_target_optical_depth() == b.get_target_optical_depth(),
+              "propagate vs interact, packet %u", k);
+        Photon c = a;
+        c.set_position(0.1 * pc, 0.2 * pc, 0.3 * pc);
+        c.set_target_optical_depth(0.);
+        const int ec = grid[7].compute_optical_depth(c, TRAVELDIRECTION_INSIDE);
+        CHECK(ec != TRAVELDIRECTION_INSIDE && c.get_target_optical_depth() > 0.,
+              "optical depth walk, packet %u", k);
+      }
+    }
     // neighbours (testDensitySubGrid_omp.cpp:1836-1874): subgrid 7 = (1,1,1)
     CHECK(grid[7].get_neighbour(TRAVELDIRECTION_INSIDE) == 7, "self neighbour");
     CHECK(grid[7].get_neighbour(TRAVELDIRECTION_CORNER_NNN) == 0, "corner neighbour");
@@ -177,6 +198,27 @@ int main() {
     const double Rs = std::cbrt(3. * 4.26e49 / (4. * M_PI * 1e16 * 2.7e-19));
     std::printf("Stromgren R/Rs = %.4f\n", R / Rs);
     CHECK(std::fabs(R / Rs - 1.) < 0.02, "Stromgren radius %.4f", R / Rs);
+    // the reference's result files: 60 bytes per cell, x_n raw
+    sg.output_neutral_fractions("/tmp/cmi_facade_neutral_fractions.dat");
+    sg.output_neutral_fractions_binary("/tmp/cmi_facade_neutral_fractions.bin");
+    {
+      FILE *f = std::fopen("/tmp/cmi_facade_neutral_fractions.dat", "rb");
+      CHECK(f != nullptr, "dat written");
+      if (f) {
+        std::fseek(f, 0, SEEK_END);
+        CHECK(std::ftell(f) == 64L * 64 * 64 * 60, "dat size");
+        std::fclose(f);
+      }
+      f = std::fopen("/tmp/cmi_facade_neutral_fractions.bin", "rb");
+      CHECK(f != nullptr, "bin written");
+      if (f) {
+        double v = -1.;
+        CHECK(std::fread(&v, sizeof(double), 1, f) == 1 &&
+                  v == sg[0].get_neutral_fraction()[0],
+              "bin first value");
+        std::fclose(f);
+      }
+    }
 
     // ---- errors come back as exceptions, not aborts ----
     bool threw = false;
