@@ -304,6 +304,10 @@ int cmi_net_set_params(cmi_net *net, const float *params);
  * are divided by it, predictions multiplied) */
 int cmi_net_set_normalisation(cmi_net *net, const float mean[2],
                               const float scale[2], float logit_scale);
+/* dilation of each layer's 3x3x3 taps (default 1, 1, 1, 1; the taps of
+ * layer l sit at offsets -d_l, 0, +d_l per axis, zero padded) */
+int cmi_net_set_dilation(cmi_net *net, const int dilation[4]);
+int cmi_net_get_dilation(const cmi_net *net, int dilation[4]);
 /* append training samples (kept on the device) */
 int cmi_net_add_samples(cmi_net *net, const float *features,
                         const float *target_logits, int nsamples,

@@ -127,6 +127,8 @@ SIGNATURES = {
     "cmi_net_set_normalisation": (C.c_int, [_vp, _fp, _fp, C.c_float]),
     "cmi_net_add_samples": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip]),
     "cmi_net_sample_count": (C.c_int, [_vp]),
+    "cmi_net_set_dilation": (C.c_int, [_vp, _ip]),
+    "cmi_net_get_dilation": (C.c_int, [_vp, _ip]),
     "cmi_net_train": (C.c_int, [_vp, C.c_int, C.c_int, _ip, C.c_float,
                                 C.c_uint64, _fp]),
     "cmi_net_loss_and_grad": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip, _fp,
@@ -515,6 +517,16 @@ class WarmStartNet:
 
     def sample_count(self):
         return self.lib.cmi_net_sample_count(self.h)
+
+    def set_dilation(self, dilation):
+        d = np.ascontiguousarray(dilation, dtype=np.int32)
+        assert d.size == 4
+        self._check(self.lib.cmi_net_set_dilation(self.h, _p(d, _ip)))
+
+    def dilation(self):
+        d = np.zeros(4, np.int32)
+        self._check(self.lib.cmi_net_get_dilation(self.h, _p(d, _ip)))
+        return d
 
     def train(self, steps, batch, crop, lr, seed=1):
         losses = np.zeros(int(steps), np.float32)

@@ -59,7 +59,7 @@ __device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
 template <int NB>
 __global__ void __launch_bounds__(256) conv3_kernel(
     const __bf16 *__restrict__ x, const __bf16 *__restrict__ w,
-    const float *__restrict__ bias, int nrun, int D, int H, int W,
+    const float *__restrict__ bias, int nrun, int D, int H, int W, int dil,
     __bf16 *__restrict__ y, float *__restrict__ yf,
     const __bf16 *__restrict__ mask, int relu) {
   const int lane = threadIdx.x & 63;
@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(256) conv3_kernel(
 #pragma unroll 1
   for (int tap = 0; tap < kTaps; ++tap) {
     const int kd = tap / 9, kh = (tap / 3) % 3, kw = tap % 3;
-    const int zd = d + kd - 1, zh = h + kh - 1, zw = w0 + vl + kw - 1;
+    const int zd = d + (kd - 1) * dil, zh = h + (kh - 1) * dil,
+              zw = w0 + vl + (kw - 1) * dil;
     bf16x8 a;
     if ((unsigned)zd < (unsigned)D && (unsigned)zh < (unsigned)H &&
         (unsigned)zw < (unsigned)W) {
@@ -122,8 +123,8 @@ __global__ void __launch_bounds__(256) conv3_kernel(
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) wgrad_kernel(
     const __bf16 *__restrict__ dy, const __bf16 *__restrict__ x, int nb_out,
-    int nrow, int rows_per_wave, int D, int H, int W, float *__restrict__ dW,
-    int NOUT) {
+    int nrow, int rows_per_wave, int D, int H, int W, int dil,
+    float *__restrict__ dW, int NOUT) {
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int nchunks = (nrow + rows_per_wave - 1) / rows_per_wave;
@@ -148,12 +149,12 @@ __global__ void __launch_bounds__(256) wgrad_kernel(
     const int b = t / D;
     const size_t vbase = (((size_t)b * D + d) * H + h) * W + w0 + 8 * kg;
     bf16x8 a, b0, b1;
-    const int zd = d + kd - 1, zh = h + kh - 1;
+    const int zd = d + (kd - 1) * dil, zh = h + (kh - 1) * dil;
     const bool rowin = (unsigned)zd < (unsigned)D && (unsigned)zh < (unsigned)H;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       a[j] = dy[(vbase + j) * kC + mb * 16 + ml];
-      const int zw = w0 + 8 * kg + j + kw - 1;
+      const int zw = w0 + 8 * kg + j + (kw - 1) * dil;
       if (rowin && (unsigned)zw < (unsigned)W) {
         const size_t vx = (((size_t)b * D + zd) * H + zh) * W + zw;
         b0[j] = x[vx * kC + ml];
@@ -374,6 +375,7 @@ struct cmi_net {
   // normalisation of the features and of the target logits
   float mean[2] = {8.f, -10.f}, scale[2] = {1.f, 0.25f};
   float out_scale = 8.f;  // logit = out_scale * network output
+  int dil[kLayers] = {1, 1, 1, 1};  // dilation of each layer's 3x3x3 taps
   // dataset (device): features [S][N][2] FP32, targets [S][N] FP32
   float *ds_feat = nullptr, *ds_tgt = nullptr;
   int ds_n = 0, ds_dim[3] = {0, 0, 0};
@@ -440,11 +442,11 @@ int forward(cmi_net *n, int B, int D, int H, int W, float *zout) {
     const float *bias = n->p + n->off_b[l];
     if (l < kLayers - 1) {
       hipLaunchKernelGGL(conv3_kernel<2>, dim3(blocks), dim3(256), 0, n->stream, n->act[l],
-                         n->wf[l], bias, nrun, D, H, W, n->act[l + 1], (float *)nullptr,
+                         n->wf[l], bias, nrun, D, H, W, n->dil[l], n->act[l + 1], (float *)nullptr,
                          (const __bf16 *)nullptr, 1);
     } else {
       hipLaunchKernelGGL(conv3_kernel<1>, dim3(blocks), dim3(256), 0, n->stream, n->act[l],
-                         n->wf[l], bias, nrun, D, H, W, (__bf16 *)nullptr, zout,
+                         n->wf[l], bias, nrun, D, H, W, n->dil[l], (__bf16 *)nullptr, zout,
                          (const __bf16 *)nullptr, 0);
     }
   }
@@ -470,13 +472,14 @@ int backward(cmi_net *n, int B, int D, int H, int W) {
     const int nb_out = NOUT / 16;
     const size_t waves = (size_t)kTaps * nb_out * nchunks;
     hipLaunchKernelGGL(wgrad_kernel, dim3(grid_for(waves, 4, 1u << 30)), dim3(256), 0, n->stream, dy,
-                       n->act[l], nb_out, nrow, rows_per_wave, D, H, W, n->g + n->off_w[l], NOUT);
+                       n->act[l], nb_out, nrow, rows_per_wave, D, H, W, n->dil[l],
+                       n->g + n->off_w[l], NOUT);
     hipLaunchKernelGGL(bgrad_kernel, dim3(grid_for(nvox, 8, 2048)), dim3(256), 0, n->stream,
                        dy, nvox, layer_cout(l), n->g + n->off_b[l]);
     if (l > 0) {
       // dx = conv(dy, flipped transposed W), masked by the ReLU of layer l-1
       hipLaunchKernelGGL(conv3_kernel<2>, dim3(blocks), dim3(256), 0, n->stream, dy, n->wb[l],
-                         (const float *)nullptr, nrun, D, H, W, dx, (float *)nullptr,
+                         (const float *)nullptr, nrun, D, H, W, n->dil[l], dx, (float *)nullptr,
                          (const __bf16 *)n->act[l], 0);
       std::swap(dy, dx);
     }
@@ -641,6 +644,20 @@ int cmi_net_add_samples(cmi_net *n, const float *feat, const float *target, int 
 }
 
 int cmi_net_sample_count(const cmi_net *n) { return n ? n->ds_n : 0; }
+
+int cmi_net_set_dilation(cmi_net *n, const int dilation[4]) {
+  if (!n || !dilation) return CMI_ERR_ARG;
+  for (int l = 0; l < kLayers; ++l)
+    if (dilation[l] < 1 || dilation[l] > 64) return nfail(n, CMI_ERR_ARG, "dilation must be 1..64");
+  for (int l = 0; l < kLayers; ++l) n->dil[l] = dilation[l];
+  return CMI_OK;
+}
+
+int cmi_net_get_dilation(const cmi_net *n, int dilation[4]) {
+  if (!n || !dilation) return CMI_ERR_ARG;
+  for (int l = 0; l < kLayers; ++l) dilation[l] = n->dil[l];
+  return CMI_OK;
+}
 
 int cmi_net_train(cmi_net *n, int steps, int batch, const int crop[3], float lr,
                   uint64_t seed, float *losses) {

@@ -33,11 +33,13 @@ def main(argv=None):
     ap.add_argument("--ncell", type=int, default=256)
     ap.add_argument("--crop", type=int, default=64)
     ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--batch", type=int, default=8)
-    ap.add_argument("--train-crop", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--train-crop", type=int, default=64)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", required=True)
+    ap.add_argument("--offsets", type=float, nargs="*",
+                    default=[0.0, -1.0, -2.0, -3.0, -4.0])
     ap.add_argument("--save-data", default=None,
                     help="also write the training crops to this .npz")
     a = ap.parse_args(argv)
@@ -54,8 +56,9 @@ def main(argv=None):
     ntrain = len(feats) - per if a.problems > 1 else len(feats)
     f_tr, t_tr = feats[:ntrain], tgts[:ntrain]
     f_va, t_va = feats[ntrain:], tgts[ntrain:]
-    mean = f_tr.reshape(-1, 2).mean(axis=0)
-    std = f_tr.reshape(-1, 2).std(axis=0)
+    # FP64 statistics: a float32 reduction over 1e8 values drifts badly
+    mean = f_tr.reshape(-1, 2).mean(axis=0, dtype=np.float64)
+    std = f_tr.reshape(-1, 2).std(axis=0, dtype=np.float64)
     net = WS.new_net(a.seed, mean, 1.0 / std)
     for i in range(0, ntrain, 64):
         net.add_samples(f_tr[i:i + 64], t_tr[i:i + 64])
@@ -80,12 +83,40 @@ def main(argv=None):
         return float(np.mean(v))
 
     vl = val_loss()
+    log("validation loss %s" % vl)
+
+    # the logit offset: iterations to convergence on a held-out problem for
+    # a few shifts toward ionized (a front predicted too close to its source
+    # costs about one iteration per cell, one predicted too far converges
+    # geometrically), the best one baked into the last bias
+    box, n, pos, L = WS.clumpy_problem(a.ncell, field_seed=999, src_seed=998,
+                                       lum_seed=997)
+    prob = WS.Problem(box, n, pos, L)
+    plain = WS.converge_plain(prob, int(a.packets), seed=31)
+    log("held-out problem, plain: %d iterations %.3f s"
+        % (plain["iterations"], plain["total_seconds"]))
+    trials = []
+    applied = 0.0
+    for delta in a.offsets:
+        WS.shift_logits(net, delta - applied)
+        applied = delta
+        w = WS.converge_warm(prob, net, int(a.packets), seed=31)
+        trials.append({"offset": delta, "iterations": w["iterations"],
+                       "seconds": w["total_seconds"], "l1": w["l1"]})
+        log("offset %+.1f: %d iterations %.3f s" % (delta, w["iterations"],
+                                                   w["total_seconds"]))
+    best = min(trials, key=lambda t: (t["iterations"], t["seconds"]))
+    WS.shift_logits(net, best["offset"] - applied)
+    prob.close()
     meta = {"problems": a.problems, "packets": a.packets, "ncell": a.ncell,
             "crop": a.crop, "crops": int(len(feats)), "train_crops": int(ntrain),
             "steps": a.steps, "batch": a.batch, "train_crop": a.train_crop,
             "lr": a.lr, "train_seconds": t_train,
             "final_train_loss": float(losses[-50:].mean()),
             "validation_loss": vl, "loss_units": "(logit / 8)^2",
+            "logit_offset": best["offset"], "offset_trials": trials,
+            "heldout_plain_iterations": plain["iterations"],
+            "heldout_plain_seconds": plain["total_seconds"],
             "problem_records": recs}
     log(json.dumps({k: v for k, v in meta.items() if k != "problem_records"}))
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)

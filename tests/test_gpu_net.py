@@ -49,7 +49,7 @@ def bf16(x):
     return x.to(torch.bfloat16).to(torch.float32)
 
 
-def torch_forward(params, feat):
+def torch_forward(params, feat, dil=(1, 1, 1, 1)):
     """feat (B, D, H, W, 2) normalised features -> logits (B, D, H, W)"""
     x = bf16(torch.tensor(feat).permute(0, 4, 1, 2, 3))
     ws = []
@@ -57,7 +57,7 @@ def torch_forward(params, feat):
         wt = bf16(torch.tensor(w)).requires_grad_(True)
         bt = torch.tensor(b).requires_grad_(True)
         ws.append((wt, bt))
-        x = torch.nn.functional.conv3d(x, wt, bt, padding=1)
+        x = torch.nn.functional.conv3d(x, wt, bt, padding=dil[i], dilation=dil[i])
         if i < 3:
             x = bf16(torch.relu(x))
     return x[:, 0], ws
@@ -81,13 +81,16 @@ def test_forward_matches_torch(net):
     assert np.abs(z - zt).max() < 2e-2 * scale, (np.abs(z - zt).max(), scale)
 
 
-def test_gradients_match_torch(net):
+@pytest.mark.parametrize("dil", [(1, 1, 1, 1), (1, 2, 4, 8)])
+def test_gradients_match_torch(net, dil):
     rng = np.random.default_rng(1)
     B = 2
     feat = rng.normal(size=(B, 16, 16, 32, 2)).astype(np.float32)
     tgt = rng.normal(size=(B, 16, 16, 32)).astype(np.float32)
+    net.set_dilation(dil)
     loss, g = net.loss_and_grad(feat, tgt)
-    zt, ws = torch_forward(split_params(net.params()), feat)
+    net.set_dilation((1, 1, 1, 1))
+    zt, ws = torch_forward(split_params(net.params()), feat, dil)
     lt = ((zt - torch.tensor(tgt)) ** 2).mean()
     lt.backward()
     assert abs(loss / lt.item() - 1) < 1e-2
