@@ -530,8 +530,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   const int T0 = G.T[0], T1 = G.T[1], T2 = G.T[2];
   const int st0 = T1 * T2, st1 = T2;
   const int max_steps = T0 + T1 + T2 + 4;
-  unsigned long long my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0,
-                     my_err = 0;
+  // per-lane counters of this launch (32 bits: a lane walks at most a few
+  // hundred thousand crossings per launch); widened for the final sums
+  uint32_t my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0, my_err = 0;
 
   for (;;) {
     if (threadIdx.x == 0) s_desc = atomicAdd(ticket, 1u);
@@ -542,7 +543,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     const int tile = D.tile;
     const int prob = G.tile_prob[tile];
     const ProbGeo &P = G.probs[prob];
-    const unsigned long long cross0 = my_cross;
+    const uint32_t cross0 = my_cross;
     // stage the tile
     {
       const float4 *src =
@@ -585,11 +586,14 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     float t = 0.f, tau = 0.f;
     uint32_t my_pid = 0, vis_cross = 0, n_reemit = 0;
 
+    float kap_c = 0.f;  // opacity of the lane's current cell (read ahead)
     for (;;) {
-      const uint64_t fin = __ballot(pending);
-      const uint64_t ready = __ballot(!live && !pending && has_nxt);
-      const uint64_t lv = __ballot(live);
-      if (__popcll(fin) + __popcll(ready) >= refill_batch || lv == 0) {
+      // refill when refill_batch lanes have finished or wait for a record,
+      // or when no lane is live; one ballot while every lane is walking
+      const uint64_t idle = __ballot(!live);
+      if (idle != 0ull &&
+          (idle == ~0ull ||
+           __popcll(__ballot(!live && (pending || has_nxt))) >= refill_batch)) {
         // ---- write out finished packets ----
         if (pending) {
           my_cross += vis_cross;
@@ -683,6 +687,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             so1 = d1 > 0.f ? st1 : -st1;
             so2 = d2 > 0.f ? 1 : -1;
             off = (l0 * T1 + l1) * T2 + l2;
+            kap_c = kapL[off];
             live = true;
           }
         }
@@ -709,7 +714,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       // ---- one crossing ----
       const float tn = fminf(tm0, fminf(tm1, tm2));
       const float seg = fmaxf(tn - t, 0.f);
-      const float kap = kapL[off];
+      const float kap = kap_c;
       const float dtau = seg * kap;
       ++vis_cross;
       if (dtau >= tau) {
@@ -781,6 +786,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       tm2 += a2 ? td2 : 0.f;
       const bool out0 = (unsigned)l0 >= (unsigned)T0, out1 = (unsigned)l1 >= (unsigned)T1,
                  out2 = (unsigned)l2 >= (unsigned)T2;
+      if (!(out0 | out1 | out2)) kap_c = kapL[off];  // the next cell, read ahead
       if (out0 | out1 | out2) {
         exit_axis = out0 ? 0 : (out1 ? 1 : 2);
         const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
@@ -820,7 +826,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       }
       if (prob_cross) {
         // crossings of this visit, per problem (one atomic per wave)
-        unsigned long long v = my_cross - cross0;
+        unsigned long long v = my_cross - cross0;  // this visit, widened
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         if (lane == 0 && v) atomicAdd(&prob_cross[prob], v);
@@ -829,20 +835,22 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     __syncthreads();
   }
   // kernel-wide counters
+  unsigned long long w_cross = my_cross, w_hops = my_hops, w_abs = my_abs, w_esc = my_esc,
+                     w_err = my_err;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    my_cross += __shfl_xor(my_cross, o, 64);
-    my_hops += __shfl_xor(my_hops, o, 64);
-    my_abs += __shfl_xor(my_abs, o, 64);
-    my_esc += __shfl_xor(my_esc, o, 64);
-    my_err += __shfl_xor(my_err, o, 64);
+    w_cross += __shfl_xor(w_cross, o, 64);
+    w_hops += __shfl_xor(w_hops, o, 64);
+    w_abs += __shfl_xor(w_abs, o, 64);
+    w_esc += __shfl_xor(w_esc, o, 64);
+    w_err += __shfl_xor(w_err, o, 64);
   }
   if (lane == 0) {
-    if (my_cross) atomicAdd(&cnt->crossings, my_cross);
-    if (my_hops) atomicAdd(&cnt->hops, my_hops);
-    if (my_abs) atomicAdd(&cnt->absorbed, my_abs);
-    if (my_esc) atomicAdd(&cnt->escaped, my_esc);
-    if (my_err) atomicAdd(&cnt->errors, my_err);
+    if (w_cross) atomicAdd(&cnt->crossings, w_cross);
+    if (w_hops) atomicAdd(&cnt->hops, w_hops);
+    if (w_abs) atomicAdd(&cnt->absorbed, w_abs);
+    if (w_esc) atomicAdd(&cnt->escaped, w_esc);
+    if (w_err) atomicAdd(&cnt->errors, w_err);
   }
 }
 
