@@ -308,6 +308,15 @@ int cmi_net_set_normalisation(cmi_net *net, const float mean[2],
  * layer l sit at offsets -d_l, 0, +d_l per axis, zero padded) */
 int cmi_net_set_dilation(cmi_net *net, const int dilation[4]);
 int cmi_net_get_dilation(const cmi_net *net, int dilation[4]);
+/* Input mode of the warm start and of cmi_net_context_features: 0 (default)
+ * feature 1 is log10 jH of the context's last transport and the output is
+ * the logit of x_n; 1 feature 1 is the logit of the context's current x_n
+ * (e.g. after a few cheap iterations) and the output is added to it. */
+int cmi_net_set_input_mode(cmi_net *net, int mode);
+int cmi_net_get_input_mode(const cmi_net *net);
+/* weight of the squared logit error where the prediction is more neutral
+ * than the target (default 1: plain mean squared error) */
+int cmi_net_set_loss_weight(cmi_net *net, float w_too_neutral);
 /* append training samples (kept on the device) */
 int cmi_net_add_samples(cmi_net *net, const float *features,
                         const float *target_logits, int nsamples,
@@ -332,6 +341,13 @@ int cmi_net_predict(cmi_net *net, const float *features, const int dim[3],
  * context must be a multiple of 16. */
 int cmi_net_warm_start(cmi_net *net, cmi_ctx *ctx, double luminosity,
                        uint64_t npackets);
+/* The warm start's rule for cells the network calls neutral: a cell whose
+ * predicted logit exceeds logit_cut starts at fallback_xn instead (the plain
+ * start, 1e-6, lets the iteration form such fronts from the ionized side,
+ * where they converge fast).  Default: no cut. */
+int cmi_net_set_start_rule(cmi_net *net, float logit_cut, double fallback_xn);
+int cmi_net_get_start_rule(const cmi_net *net, float *logit_cut,
+                           double *fallback_xn);
 /* The same features of a context's current J, as [D][H][W][2] FP32 on the
  * host (for building training sets from contexts). */
 int cmi_net_context_features(cmi_net *net, cmi_ctx *ctx, double luminosity,

@@ -128,6 +128,12 @@ SIGNATURES = {
     "cmi_net_add_samples": (C.c_int, [_vp, _fp, _fp, C.c_int, _ip]),
     "cmi_net_sample_count": (C.c_int, [_vp]),
     "cmi_net_set_dilation": (C.c_int, [_vp, _ip]),
+    "cmi_net_set_loss_weight": (C.c_int, [_vp, C.c_float]),
+    "cmi_net_set_input_mode": (C.c_int, [_vp, C.c_int]),
+    "cmi_net_get_input_mode": (C.c_int, [_vp]),
+    "cmi_net_set_start_rule": (C.c_int, [_vp, C.c_float, C.c_double]),
+    "cmi_net_get_start_rule": (C.c_int, [_vp, C.POINTER(C.c_float),
+                                         C.POINTER(C.c_double)]),
     "cmi_net_get_dilation": (C.c_int, [_vp, _ip]),
     "cmi_net_train": (C.c_int, [_vp, C.c_int, C.c_int, _ip, C.c_float,
                                 C.c_uint64, _fp]),
@@ -522,6 +528,26 @@ class WarmStartNet:
         d = np.ascontiguousarray(dilation, dtype=np.int32)
         assert d.size == 4
         self._check(self.lib.cmi_net_set_dilation(self.h, _p(d, _ip)))
+
+    def set_input_mode(self, mode):
+        self._check(self.lib.cmi_net_set_input_mode(self.h, int(mode)))
+
+    def input_mode(self):
+        return self.lib.cmi_net_get_input_mode(self.h)
+
+    def set_loss_weight(self, w_too_neutral):
+        self._check(self.lib.cmi_net_set_loss_weight(self.h, float(w_too_neutral)))
+
+    def set_start_rule(self, logit_cut, fallback_xn=1e-6):
+        self._check(self.lib.cmi_net_set_start_rule(self.h, float(logit_cut),
+                                                    float(fallback_xn)))
+
+    def start_rule(self):
+        c = C.c_float()
+        f = C.c_double()
+        self._check(self.lib.cmi_net_get_start_rule(self.h, C.byref(c),
+                                                    C.byref(f)))
+        return c.value, f.value
 
     def dilation(self):
         d = np.zeros(4, np.int32)

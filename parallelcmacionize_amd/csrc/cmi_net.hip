@@ -194,19 +194,22 @@ __global__ void bgrad_kernel(const __bf16 *__restrict__ dy, size_t nvox, int nc,
   if (threadIdx.x < nc) atomicAdd(&db[threadIdx.x], sh[threadIdx.x]);
 }
 
-// Mean squared error on logits: loss += sum (z - t)^2 / n; dy[v][0] =
-// 2 (z - t) / n, other channels 0.
+// Weighted mean squared error on logits: with e = z - t and w = w_pos for
+// e > 0 (a prediction more neutral than the target), 1 otherwise,
+// loss += sum w e^2 / n; dy[v][0] = 2 w e / n, other channels 0.
 __global__ void mse_kernel(const float *__restrict__ z, const float *__restrict__ tgt,
-                           size_t n, __bf16 *__restrict__ dy, float *__restrict__ loss) {
+                           size_t n, float w_pos, __bf16 *__restrict__ dy,
+                           float *__restrict__ loss) {
   __shared__ float sh[4];
   float s = 0.f;
   const float inv = 1.f / (float)n;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
        v += (size_t)gridDim.x * blockDim.x) {
     const float e = z[v] - tgt[v];
-    s += e * e;
+    const float w = e > 0.f ? w_pos : 1.f;
+    s += w * e * e;
     __bf16 *o = dy + v * kC;
-    o[0] = f2bf(2.f * e * inv);
+    o[0] = f2bf(2.f * w * e * inv);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
@@ -294,20 +297,34 @@ __device__ __forceinline__ size_t lattice_addr(const GridDev &g, int gx, int gy,
 
 constexpr double kJHFloor = 1e-12;  // s^-1, added before the logarithm
 
-// Features of a context's cells in lattice order: log10 n_H and
-// log10(jconv * J + floor); normalised BF16 into act (channels 0, 1 of 32)
-// and/or raw FP32 into feat ([v][2]).
+// logit of x_n clamped to [1e-7, 1 - 1/150]: above that every cell is
+// opaque (a 256^3-workload cell of neutral gas has optical depth ~76), and
+// the clamp keeps the Monte Carlo noise of near-neutral cells (1 - x ~ 1e-6)
+// out of the features and targets
+constexpr double kXnHigh = 1. - 1. / 150.;
+__device__ __forceinline__ double clamped_logit(double x) {
+  x = fmin(fmax(x, 1e-7), kXnHigh);
+  return log(x / (1. - x));
+}
+
+// Features of a context's cells in lattice order: log10 n_H and, in input
+// mode 0, log10(jconv * J + floor) (the photoionization rate of the last
+// transport), in mode 1 the logit of the current x_n; normalised BF16 into
+// act (channels 0, 1 of 32) and/or raw FP32 into feat ([v][2]).
 __global__ void ctx_features_kernel(GridDev g, const double *__restrict__ n,
-                                    const double *__restrict__ J, int D, int H, int W,
-                                    double jconv, float m0, float s0, float m1, float s1,
-                                    __bf16 *__restrict__ act, float *__restrict__ feat) {
+                                    const double *__restrict__ J,
+                                    const double *__restrict__ xn, int mode, int D, int H,
+                                    int W, double jconv, float m0, float s0, float m1,
+                                    float s1, __bf16 *__restrict__ act,
+                                    float *__restrict__ feat) {
   const size_t nv = (size_t)D * H * W;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
        v += (size_t)gridDim.x * blockDim.x) {
     const int gz = (int)(v % W), gy = (int)((v / W) % H), gx = (int)(v / ((size_t)W * H));
     const size_t a = lattice_addr(g, gx, gy, gz);
     const float f0 = (float)log10(n[a]);
-    const float f1 = (float)log10(jconv * J[a] + kJHFloor);
+    const float f1 = mode == 1 ? (float)clamped_logit(xn[a])
+                               : (float)log10(jconv * J[a] + kJHFloor);
     if (feat) {
       feat[2 * v] = f0;
       feat[2 * v + 1] = f1;
@@ -325,9 +342,13 @@ __global__ void ctx_features_kernel(GridDev g, const double *__restrict__ n,
   }
 }
 
-// Logits (lattice order) -> x_n of the context, J = 0, FP32 opacity.
-__global__ void ctx_apply_kernel(GridDev g, const float *__restrict__ z, int D, int H, int W,
-                                 float out_scale, const double *__restrict__ n,
+// Network outputs (lattice order) -> x_n of the context, J = 0, FP32
+// opacity.  Mode 0: the output is the logit of x_n; mode 1: it is added to
+// the logit of the current x_n.  A cell whose logit exceeds `cut`
+// (predicted neutral) starts at `fallback` instead.
+__global__ void ctx_apply_kernel(GridDev g, const float *__restrict__ z, int mode, int D,
+                                 int H, int W, float out_scale, float cut, double fallback,
+                                 const double *__restrict__ n,
                                  double *__restrict__ xn, double *__restrict__ J,
                                  float *__restrict__ kappa, double kscale) {
   const size_t nv = (size_t)D * H * W;
@@ -335,9 +356,11 @@ __global__ void ctx_apply_kernel(GridDev g, const float *__restrict__ z, int D, 
        v += (size_t)gridDim.x * blockDim.x) {
     const int gz = (int)(v % W), gy = (int)((v / W) % H), gx = (int)(v / ((size_t)W * H));
     const size_t a = lattice_addr(g, gx, gy, gz);
-    const double logit = (double)z[v] * (double)out_scale;
+    const double logit =
+        (double)z[v] * (double)out_scale + (mode == 1 ? clamped_logit(xn[a]) : 0.);
     double x = 1. / (1. + exp(-logit));
     x = fmin(fmax(x, 1e-7), 1.);
+    if (logit > (double)cut) x = fallback;
     xn[a] = x;
     J[a] = 0.;
     if (kappa) kappa[a] = (float)(kscale * n[a] * x);
@@ -375,7 +398,13 @@ struct cmi_net {
   // normalisation of the features and of the target logits
   float mean[2] = {8.f, -10.f}, scale[2] = {1.f, 0.25f};
   float out_scale = 8.f;  // logit = out_scale * network output
+  // start rule of the warm start: cells predicted more neutral than logit
+  // `cut` start at `fallback` (default: no cut)
+  float cut = 3.0e38f;
+  double fallback = 1e-6;
   int dil[kLayers] = {1, 1, 1, 1};  // dilation of each layer's 3x3x3 taps
+  float w_pos = 1.f;  // loss weight of predictions more neutral than the target
+  int mode = 0;       // input mode of the warm start (0: jH, 1: refine x_n)
   // dataset (device): features [S][N][2] FP32, targets [S][N] FP32
   float *ds_feat = nullptr, *ds_tgt = nullptr;
   int ds_n = 0, ds_dim[3] = {0, 0, 0};
@@ -466,7 +495,7 @@ int backward(cmi_net *n, int B, int D, int H, int W) {
   hipLaunchKernelGGL(zero_bf16_kernel, dim3(grid_for(nvox * kC, 256, 4096)), dim3(256), 0,
                      n->stream, dy, nvox * kC);
   hipLaunchKernelGGL(mse_kernel, dim3(grid_for(nvox, 256, 1024)), dim3(256), 0, n->stream,
-                     n->z, n->tgt, nvox, dy, n->loss);
+                     n->z, n->tgt, nvox, n->w_pos, dy, n->loss);
   for (int l = kLayers - 1; l >= 0; --l) {
     const int NOUT = nout_of(l);
     const int nb_out = NOUT / 16;
@@ -653,6 +682,35 @@ int cmi_net_set_dilation(cmi_net *n, const int dilation[4]) {
   return CMI_OK;
 }
 
+int cmi_net_set_input_mode(cmi_net *n, int mode) {
+  if (!n || (mode != 0 && mode != 1)) return CMI_ERR_ARG;
+  n->mode = mode;
+  return CMI_OK;
+}
+
+int cmi_net_get_input_mode(const cmi_net *n) { return n ? n->mode : -1; }
+
+int cmi_net_set_loss_weight(cmi_net *n, float w_too_neutral) {
+  if (!n || !(w_too_neutral > 0.f)) return CMI_ERR_ARG;
+  n->w_pos = w_too_neutral;
+  return CMI_OK;
+}
+
+int cmi_net_set_start_rule(cmi_net *n, float logit_cut, double fallback_xn) {
+  if (!n || !(fallback_xn > 0. && fallback_xn <= 1.) || logit_cut != logit_cut)
+    return CMI_ERR_ARG;
+  n->cut = logit_cut;
+  n->fallback = fallback_xn;
+  return CMI_OK;
+}
+
+int cmi_net_get_start_rule(const cmi_net *n, float *logit_cut, double *fallback_xn) {
+  if (!n || !logit_cut || !fallback_xn) return CMI_ERR_ARG;
+  *logit_cut = n->cut;
+  *fallback_xn = n->fallback;
+  return CMI_OK;
+}
+
 int cmi_net_get_dilation(const cmi_net *n, int dilation[4]) {
   if (!n || !dilation) return CMI_ERR_ARG;
   for (int l = 0; l < kLayers; ++l) dilation[l] = n->dil[l];
@@ -815,14 +873,17 @@ int cmi_net_warm_start(cmi_net *n, cmi_ctx *c, double luminosity, uint64_t npack
   if (rc != CMI_OK) return rc;
   NET_TRY(n, hipStreamSynchronize(c->stream));  // the first pass has finished
   hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0,
-                     n->stream, c->g, (const double *)c->f.n, (const double *)c->f.J, D, H, W,
+                     n->stream, c->g, (const double *)c->f.n, (const double *)c->f.J,
+                     (const double *)c->f.xn, n->mode, D, H, W,
                      context_jconv(c, luminosity, npackets), n->mean[0], n->scale[0],
                      n->mean[1], n->scale[1], n->act[0], (float *)nullptr);
   rc = forward(n, 1, D, H, W, n->z);
   if (rc != CMI_OK) return rc;
   const bool fp32 = c->precision == CMI_PREC_FP32;
   hipLaunchKernelGGL(ctx_apply_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0, n->stream,
-                     c->g, (const float *)n->z, D, H, W, n->out_scale, (const double *)c->f.n,
+                     c->g, (const float *)n->z, n->mode, D, H, W, n->out_scale, n->cut,
+                     n->fallback,
+                     (const double *)c->f.n,
                      c->f.xn, c->f.J, fp32 ? c->f.kappa : (float *)nullptr,
                      c->sigma * c->g.cell_size[0]);
   NET_TRY(n, hipGetLastError());
@@ -842,7 +903,8 @@ int cmi_net_context_features(cmi_net *n, cmi_ctx *c, double luminosity, uint64_t
   NET_TRY(n, hipStreamSynchronize(c->stream));
   NET_TRY(n, hipMalloc(&d, nv * 2 * sizeof(float)));
   hipLaunchKernelGGL(ctx_features_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0,
-                     n->stream, c->g, (const double *)c->f.n, (const double *)c->f.J, D, H, W,
+                     n->stream, c->g, (const double *)c->f.n, (const double *)c->f.J,
+                     (const double *)c->f.xn, n->mode, D, H, W,
                      context_jconv(c, luminosity, npackets), 0.f, 1.f, 0.f, 1.f,
                      (__bf16 *)nullptr, d);
   hipError_t e = hipGetLastError();

@@ -6,10 +6,14 @@
 Training problems are 256^3 clumpy media like the 256^3 clumpy workload
 (same box, statistics and source ranges; field and source seeds 1000+,
 2000+, 3000+ so the workload itself is never seen), each converged by the
-plain iteration (1e8 packets per iteration, mean |dx_n| < 1e-3) and cut into
-64 crops of 64^3.  The last problem's crops are held out for validation.
-Training: Adam on random 32^3 crops, BF16 MFMA kernels with FP32
-accumulation, all on the GPU.
+plain iteration (1e8 packets per iteration, common random numbers, mean
+|dx_n| < 2e-4) and cut into 64 crops of 64^3.  Inputs: log10 n_H and log10
+jH after `passes` cheap transports of `pass_packets` packets.  Training: Adam
+on whole 64^3 crops, BF16 MFMA kernels with FP32 accumulation, all on the
+GPU, with predictions more neutral than the target weighted `w_neutral` in
+the loss.  Then a held-out 256^3 problem chooses the start rule (cut of
+predicted-neutral cells) by wall time to convergence at the plain
+iteration's accuracy.
 """
 import argparse
 import json
@@ -20,6 +24,7 @@ import time
 import numpy as np
 
 from . import warmstart as WS
+from . import workloads as W
 
 
 def log(*a):
@@ -32,95 +37,111 @@ def main(argv=None):
     ap.add_argument("--packets", type=float, default=1e8)
     ap.add_argument("--ncell", type=int, default=256)
     ap.add_argument("--crop", type=int, default=64)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--passes", type=int, default=WS.PASSES)
+    ap.add_argument("--pass-packets", type=float, default=WS.PASS_PACKETS)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--train-crop", type=int, default=64)
-    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--w-neutral", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cuts", type=float, nargs="*", default=[3.0e38, -4.0])
+    ap.add_argument("--mode", type=int, default=1,
+                    help="0: inputs log n_H, log jH of the last cheap pass; "
+                         "1: log n_H and the cheap iterate's logit x_n, the "
+                         "network adds a correction to that logit")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--offsets", type=float, nargs="*",
-                    default=[0.0, -1.0, -2.0, -3.0, -4.0])
-    ap.add_argument("--save-data", default=None,
-                    help="also write the training crops to this .npz")
     a = ap.parse_args(argv)
+    npk = int(a.pass_packets)
+    N = int(a.packets)
 
     t0 = time.time()
-    feats, tgts, recs = WS.make_training_set(range(a.problems), int(a.packets),
-                                             crop=a.crop, ncell=a.ncell,
-                                             log=log)
+    feats, tgts, recs = WS.make_training_set(range(a.problems), N, crop=a.crop,
+                                             ncell=a.ncell, log=log,
+                                             passes=a.passes, first_packets=npk,
+                                             mode=a.mode)
     log("training set: %d crops of %d^3 in %.1f s" % (len(feats), a.crop,
                                                       time.time() - t0))
-    if a.save_data:
-        np.savez(a.save_data, features=feats, targets=tgts)
-    per = (a.ncell // a.crop) ** 3
-    ntrain = len(feats) - per if a.problems > 1 else len(feats)
-    f_tr, t_tr = feats[:ntrain], tgts[:ntrain]
-    f_va, t_va = feats[ntrain:], tgts[ntrain:]
     # FP64 statistics: a float32 reduction over 1e8 values drifts badly
-    mean = f_tr.reshape(-1, 2).mean(axis=0, dtype=np.float64)
-    std = f_tr.reshape(-1, 2).std(axis=0, dtype=np.float64)
+    mean = feats.reshape(-1, 2).mean(axis=0, dtype=np.float64)
+    std = feats.reshape(-1, 2).std(axis=0, dtype=np.float64)
     net = WS.new_net(a.seed, mean, 1.0 / std)
-    for i in range(0, ntrain, 64):
-        net.add_samples(f_tr[i:i + 64], t_tr[i:i + 64])
+    net.set_loss_weight(a.w_neutral)
+    net.set_input_mode(a.mode)
+    net.passes, net.pass_packets = a.passes, npk
+    if a.mode == 1:
+        # the correction starts at zero: the warm start begins as the cheap
+        # iterate itself
+        p = net.params()
+        p[WS.LAST_LAYER:] = 0.0
+        net.set_params(p)
+    for i in range(0, len(feats), 64):
+        net.add_samples(feats[i:i + 64], tgts[i:i + 64])
+    del feats, tgts
+    # held-out problem: the plain iteration's time and accuracy against its
+    # own tightly converged fixed point
+    box, n, pos, L = WS.clumpy_problem(a.ncell, field_seed=999, src_seed=998,
+                                       lum_seed=997)
+    prob = WS.Problem(box, n, pos, L)
+    prob.first_pass(seed=1)                        # builds the engine
+    WS.converge_plain(prob, N, seed=31, tol=1e-5, max_it=60)
+    x_ref = prob.xn_lattice()
+    plain = WS.converge_plain(prob, N, seed=31)
+    x_p = prob.xn_lattice()
+    err_plain = float(np.abs(x_p - x_ref).mean())
+    log("held-out problem, plain: %d iterations %.3f s, error %.2e"
+        % (plain["iterations"], plain["total_seconds"], err_plain))
+
+    def evaluate(label):
+        w = WS.converge_warm(prob, net, N, seed=31)
+        x_w = prob.xn_lattice()
+        err = float(np.abs(x_w - x_ref).mean())
+        log("%s: %d iterations %.3f s, error %.2e" % (label, w["iterations"],
+                                                     w["total_seconds"], err))
+        return {"iterations": w["iterations"], "seconds": w["total_seconds"],
+                "error": err, "l1": w["l1"]}
+
+    untrained = evaluate("untrained network") if a.mode == 1 else None
     t1 = time.time()
     crop = [a.train_crop] * 3
-    phases = [(0.6, a.lr), (0.3, a.lr * 0.3), (0.1, a.lr * 0.1)]
+    phases = [(0.7, a.lr), (0.2, a.lr * 0.2), (0.1, a.lr * 0.05)]
     losses = []
-    for frac, lr in phases:
+    for k, (frac, lr) in enumerate(phases):
         n = max(1, int(round(frac * a.steps)))
-        l = net.train(n, a.batch, crop, lr, seed=a.seed + len(losses))
+        l = net.train(n, a.batch, crop, lr, seed=a.seed + k)
         losses.append(l)
         log("trained %d steps at lr %.1e: loss %.4f -> %.4f"
             % (n, lr, l[:20].mean(), l[-20:].mean()))
     losses = np.concatenate(losses)
     t_train = time.time() - t1
 
-    def val_loss():
-        if len(f_va) == 0:
-            return None
-        v = [net.loss_and_grad(f_va[i:i + 1], t_va[i:i + 1])[0]
-             for i in range(len(f_va))]
-        return float(np.mean(v))
-
-    vl = val_loss()
-    log("validation loss %s" % vl)
-
-    # the logit offset: iterations to convergence on a held-out problem for
-    # a few shifts toward ionized (a front predicted too close to its source
-    # costs about one iteration per cell, one predicted too far converges
-    # geometrically), the best one baked into the last bias
-    box, n, pos, L = WS.clumpy_problem(a.ncell, field_seed=999, src_seed=998,
-                                       lum_seed=997)
-    prob = WS.Problem(box, n, pos, L)
-    plain = WS.converge_plain(prob, int(a.packets), seed=31)
-    log("held-out problem, plain: %d iterations %.3f s"
-        % (plain["iterations"], plain["total_seconds"]))
+    # each start rule on the held-out problem
     trials = []
-    applied = 0.0
-    for delta in a.offsets:
-        WS.shift_logits(net, delta - applied)
-        applied = delta
-        w = WS.converge_warm(prob, net, int(a.packets), seed=31)
-        trials.append({"offset": delta, "iterations": w["iterations"],
-                       "seconds": w["total_seconds"], "l1": w["l1"]})
-        log("offset %+.1f: %d iterations %.3f s" % (delta, w["iterations"],
-                                                   w["total_seconds"]))
-    best = min(trials, key=lambda t: (t["iterations"], t["seconds"]))
-    WS.shift_logits(net, best["offset"] - applied)
+    for cut in a.cuts:
+        net.set_start_rule(cut, 1e-6)
+        r = evaluate("cut %g" % cut)
+        r["cut"] = cut
+        trials.append(r)
+    ok = [t for t in trials if t["error"] <= 1.25 * err_plain] or trials
+    best = min(ok, key=lambda t: t["seconds"])
+    net.set_start_rule(best["cut"], 1e-6)
     prob.close()
     meta = {"problems": a.problems, "packets": a.packets, "ncell": a.ncell,
-            "crop": a.crop, "crops": int(len(feats)), "train_crops": int(ntrain),
-            "steps": a.steps, "batch": a.batch, "train_crop": a.train_crop,
-            "lr": a.lr, "train_seconds": t_train,
+            "crop": a.crop, "crops": a.problems * (a.ncell // a.crop) ** 3,
+            "passes": a.passes, "pass_packets": npk, "steps": a.steps,
+            "batch": a.batch, "train_crop": a.train_crop, "lr": a.lr,
+            "w_neutral": a.w_neutral, "train_seconds": t_train,
             "final_train_loss": float(losses[-50:].mean()),
-            "validation_loss": vl, "loss_units": "(logit / 8)^2",
-            "logit_offset": best["offset"], "offset_trials": trials,
+            "loss_units": "(logit / 8)^2, weighted",
+            "start_cut": best["cut"], "start_trials": trials,
             "heldout_plain_iterations": plain["iterations"],
             "heldout_plain_seconds": plain["total_seconds"],
+            "heldout_plain_error": err_plain, "input_mode": a.mode,
+            "heldout_untrained": untrained,
             "problem_records": recs}
     log(json.dumps({k: v for k, v in meta.items() if k != "problem_records"}))
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-    WS.save_net(net, a.out, meta)
+    WS.save_net(net, a.out, meta, passes=a.passes, pass_packets=npk)
     net.close()
 
 

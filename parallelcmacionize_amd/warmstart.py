@@ -22,7 +22,14 @@ from . import (ENGINE_TILES, PREC_FP32, Context, WarmStartNet)
 from . import workloads as W
 
 FIRST_PASS_PACKETS = 1_000_000
-XN_CLAMP = 1e-7          # targets use logit of x_n clamped to [1e-7, 1 - 1e-7]
+# the trained network's inputs come from PASSES cheap transports of
+# PASS_PACKETS packets (the same packets each time, the ionization update in
+# between): the J of the last one carries the shadowing of the ionized
+# regions, which a single pass through the x_n = 1e-6 medium cannot
+PASSES = 5
+PASS_PACKETS = 10_000_000
+XN_CLAMP = 1e-7          # targets use logit of x_n clamped to [1e-7, 1 - 1/150]
+XN_HIGH = 1.0 - 1.0 / 150.0   # above it every cell is opaque (tau ~ 76 per cell)
 LOGIT_SCALE = 8.0        # the network outputs logit / 8
 FEATURE_MEAN = (8.0, -7.0)
 FEATURE_SCALE = (2.0, 0.5)
@@ -31,7 +38,7 @@ DILATION = (1, 2, 4, 8)
 
 
 def logit(x):
-    x = np.clip(np.asarray(x, dtype=np.float64), XN_CLAMP, 1.0 - XN_CLAMP)
+    x = np.clip(np.asarray(x, dtype=np.float64), XN_CLAMP, XN_HIGH)
     return np.log(x / (1.0 - x))
 
 
@@ -75,9 +82,18 @@ class Problem:
         _, x = self.ctx.get_fields()
         return W.subgrid_major_to_global(x, self.ncell, self.nsub)
 
-    def first_pass(self, seed, npackets=FIRST_PASS_PACKETS):
-        """One transport through the current medium; leaves J in place."""
-        return self.ctx.transport(npackets, seed, 0)
+    def first_pass(self, seed, npackets=FIRST_PASS_PACKETS, passes=1,
+                   update_last=False):
+        """`passes` cheap transports of npackets packets (the same packets
+        each time), with the ionization update between them; the J of the
+        last one is left in place (the input of mode 0), or with
+        update_last the last update is done too (x_n is the input of
+        mode 1)."""
+        for k in range(passes):
+            s = self.ctx.transport(npackets, seed, 0)
+            if k + 1 < passes or update_last:
+                self.ctx.update_ionization(self.L, npackets, self.alpha, form=1)
+        return s
 
     def iterate(self, npackets, seed, tol=1e-3, max_it=40, it0=1,
                 min_it=1, frozen=False):
@@ -121,12 +137,17 @@ def converge_plain(prob, npackets, seed, tol=1e-3, max_it=40, frozen=True):
 
 
 def converge_warm(prob, net, npackets, seed, tol=1e-3, max_it=40,
-                  first_packets=FIRST_PASS_PACKETS, frozen=True):
-    """First pass + network + plain iterations; the wall time includes the
-    first pass and the inference."""
+                  first_packets=None, frozen=True, passes=None):
+    """First pass(es) + network + plain iterations; the wall time includes
+    the first passes and the inference."""
+    if first_packets is None:
+        first_packets = getattr(net, "pass_packets", FIRST_PASS_PACKETS)
+    if passes is None:
+        passes = getattr(net, "passes", 1)
     prob.reset()
     t0 = time.perf_counter()
-    prob.first_pass(seed ^ 0x5A5A5A5A, first_packets)
+    prob.first_pass(seed ^ 0x5A5A5A5A, first_packets, passes,
+                    update_last=net.input_mode() == 1)
     net.warm_start(prob.ctx, prob.L, first_packets)
     t1 = time.perf_counter()
     r = prob.iterate(npackets, seed, tol=tol, max_it=max_it, frozen=frozen,
@@ -149,20 +170,23 @@ def crops(a, size):
 
 def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
                       nsrc=8, tol=2e-4, max_it=40, extra_it=0, log=None,
-                      feature_net=None):
+                      feature_net=None, passes=1,
+                      first_packets=FIRST_PASS_PACKETS, mode=0):
     """Converge one clumpy problem per seed with the plain method and cut
     (features, target logits) crops.  Returns float32 arrays
     (S, crop, crop, crop, 2) and (S, crop, crop, crop) plus per-problem
     records."""
     net = feature_net or WarmStartNet(seed=1)
+    net.set_input_mode(mode)
     feats, tgts, recs = [], [], []
     for s in seeds:
         box, n, pos, L = clumpy_problem(ncell, box_side_pc, nsrc,
                                         field_seed=1000 + s, src_seed=2000 + s,
                                         lum_seed=3000 + s)
         prob = Problem(box, n, pos, L)
-        prob.first_pass(seed=7000 + s)
-        f = net.context_features(prob.ctx, prob.L, FIRST_PASS_PACKETS)
+        prob.first_pass(seed=7000 + s, npackets=first_packets, passes=passes,
+                        update_last=mode == 1)
+        f = net.context_features(prob.ctx, prob.L, first_packets)
         prob.reset()
         r = prob.iterate(npackets, seed=5000 + s, tol=tol, max_it=max_it,
                          frozen=True, min_it=MIN_IT)
@@ -170,6 +194,8 @@ def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
             prob.iterate(npackets, seed=6000 + s, tol=0.0, max_it=extra_it,
                          it0=100)
         t = logit(prob.xn_lattice()).astype(np.float32)
+        if mode == 1:
+            t = t - f[..., 1]   # the correction to the cheap iterate's logit
         feats += crops(f, crop)
         tgts += crops(t, crop)
         recs.append({"seed": int(s), "iterations": r["iterations"],
@@ -186,7 +212,8 @@ def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
 
 # offset of the last layer's bias in the master parameter vector (layers of
 # W[27][NOUT][32] + b[NOUT], NOUT = 32, 32, 32, 16)
-LAST_BIAS = 3 * (27 * 32 * 32 + 32) + 27 * 16 * 32
+LAST_LAYER = 3 * (27 * 32 * 32 + 32)
+LAST_BIAS = LAST_LAYER + 27 * 16 * 32
 
 
 def shift_logits(net, delta):
@@ -204,11 +231,14 @@ def new_net(seed=1, mean=FEATURE_MEAN, scale=FEATURE_SCALE, dilation=DILATION):
     return net
 
 
-def save_net(net, path, meta=None):
+def save_net(net, path, meta=None, passes=1, pass_packets=FIRST_PASS_PACKETS):
     mean, scale = getattr(net, "norm", (FEATURE_MEAN, FEATURE_SCALE))
     np.savez(path, params=net.params(), feature_mean=np.asarray(mean),
+             passes=passes, pass_packets=pass_packets,
+             input_mode=net.input_mode(),
              feature_scale=np.asarray(scale), logit_scale=LOGIT_SCALE,
              dilation=net.dilation(),
+             start_rule=np.array(net.start_rule()),
              meta=np.array(json.dumps(meta or {})))
 
 
@@ -219,5 +249,11 @@ def load_net(path):
                           float(d["logit_scale"]))
     if "dilation" in d:
         net.set_dilation(d["dilation"])
+    if "start_rule" in d:
+        net.set_start_rule(float(d["start_rule"][0]), float(d["start_rule"][1]))
+    if "input_mode" in d:
+        net.set_input_mode(int(d["input_mode"]))
+    net.passes = int(d["passes"]) if "passes" in d else 1
+    net.pass_packets = int(d["pass_packets"]) if "pass_packets" in d else FIRST_PASS_PACKETS
     net.set_params(d["params"])
     return net
