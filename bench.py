@@ -239,8 +239,18 @@ def main():
     box, n, pos, L = make_workload(ncell, nsub)
     prec = cmi.PREC_FP32 if args.prec == "fp32" else cmi.PREC_FP64
     ctx = cmi.Context(box, [ncell] * 3, nsub, prec)
-    ctx.set_engine(cmi.ENGINE_WALK if args.engine == "walk"
-                   else cmi.ENGINE_TILES)
+    from parallelcmacionize_amd import autotune as A
+    tuned = A.load_cache(os.path.join(ROOT, "parallelcmacionize_amd", "data",
+                                      "tuning.json")).get("clumpy256")
+    if args.engine == "walk":
+        ctx.set_engine(cmi.ENGINE_WALK)
+    elif tuned and ncell == 256 and nsub[0] == 4:
+        ctx.set_engine(cmi.ENGINE_TILES, tile=[tuned["tile"]] * 3,
+                       block_threads=tuned["block_threads"],
+                       refill_batch=tuned["refill_batch"],
+                       chunk_min=tuned["chunk"])
+    else:
+        ctx.set_engine(cmi.ENGINE_TILES)
     ctx.set_fields(n, np.full(n.size, W.XN0))
     ctx.set_sources(pos, L)
     Ltot = ctx.total_luminosity()
@@ -289,6 +299,7 @@ def main():
                          "ionization update" % (ncell, nsub[0],
                                                 ncell // nsub[0], N)),
             "engine": args.engine,
+            "tuning": tuned if (args.engine == "tiles" and tuned) else "defaults",
             "traversal": args.prec,
             "deposits": "f64 atomics",
             "cells": ncell ** 3,

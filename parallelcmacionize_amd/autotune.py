@@ -15,7 +15,7 @@ import statistics
 
 DEFAULT = {"tile": 16, "block_threads": 512, "refill_batch": 32, "chunk": 8192}
 SPACE = {
-    "tile": [8, 16, 32],
+    "tile": [8, 16],      # a 32^3 tile needs 384 KiB of LDS
     "block_threads": [256, 512],
     "refill_batch": [8, 16, 32, 64],
     "chunk": [4096, 8192, 16384, 32768],
@@ -41,7 +41,12 @@ def coordinate_descent(evaluate, space=None, start=None, passes=2,
     def measure(cfg):
         k = _key(cfg)
         if k not in seen:
-            ts = [evaluate(dict(cfg)) for _ in range(repeats)]
+            try:
+                ts = [evaluate(dict(cfg)) for _ in range(repeats)]
+            except Exception as e:  # an infeasible configuration
+                if log:
+                    log("autotune %s: rejected (%s)" % (cfg, e))
+                ts = [float("inf")]
             seen[k] = statistics.median(ts)
             trials.append({**cfg, "seconds": seen[k], "samples": ts})
             if log:
@@ -86,9 +91,7 @@ def tune_context(prob, npackets, seed=1, iteration=3, space=None,
                  passes=2, repeats=3, log=None):
     """Tune the transport of one context (a warmstart.Problem) from its
     current field state, which is restored before every timed transport."""
-    import numpy as np
     from . import ENGINE_TILES
-    from . import workloads as W
 
     _, x = prob.ctx.get_fields()
     sc = [n // s for n, s in zip(prob.ncell, prob.nsub)]
@@ -140,3 +143,64 @@ def tune_problem_set(make_set, npackets, seed=1, iteration=3, space=None,
         for b in sets.values():
             b.close()
     return best, t, trials
+
+
+def main(argv=None):
+    """python -m parallelcmacionize_amd.autotune --workload batched64 \\
+           --out parallelcmacionize_amd/data/tuning.json"""
+    import argparse
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["batched64", "clumpy256"],
+                    default="batched64")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--repeats", type=int, default=3)
+    a = ap.parse_args(argv)
+
+    def log(*m):
+        print(*m, flush=True)
+
+    t0 = time.time()
+    if a.workload == "batched64":
+        from . import batched as B
+        spec = B.problem_set_spec()
+        fields = B.build_fields(spec)
+        ps = B.ProblemSet(spec, fields)
+        for it in range(3):                      # a mid-convergence state
+            ps.iteration(seed=1, iteration=it)
+        ps.snapshot()
+        state = ps.xn_state
+        npk = ps.npackets
+        ps.close()
+
+        def make(cfg):
+            p = B.ProblemSet(spec, fields, tile=cfg["tile"])
+            p.xn_state = state
+            return p
+        space = dict(SPACE)
+        space["tile"] = [8, 16]                  # must divide 48
+        best, t, trials = tune_problem_set(make, npk, seed=3, iteration=7,
+                                           space=space, passes=a.passes,
+                                           repeats=a.repeats, log=log)
+    else:
+        from . import warmstart as WS
+        box, n, pos, L = WS.clumpy_problem()
+        prob = WS.Problem(box, n, pos, L)
+        prob.iterate(100_000_000, seed=1, tol=0.0, max_it=3)
+        best, t, trials = tune_context(prob, 100_000_000, seed=3, iteration=7,
+                                       passes=a.passes, repeats=a.repeats,
+                                       log=log)
+        prob.close()
+    dev = "MI355X (gfx950)"
+    cache = load_cache(a.out)
+    cache[a.workload] = dict(best)
+    cache[a.workload + "_record"] = {"device": dev, "seconds": t,
+                                     "trials": trials,
+                                     "tuning_seconds": time.time() - t0}
+    save_cache(a.out, cache)
+    log("best", best, "%.3f ms" % (1e3 * t))
+
+
+if __name__ == "__main__":
+    main()

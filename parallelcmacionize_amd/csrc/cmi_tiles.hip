@@ -1034,6 +1034,7 @@ static std::vector<ProbGeo> ctx_problems(const cmi_ctx *c) {
 }
 
 constexpr int kReemitRoundCap = 100000;
+constexpr size_t kLdsBytes = 160 * 1024;  // LDS of one CU on gfx950
 
 static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
   TileGeo G{};
@@ -1225,6 +1226,8 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
                       PacketOut po, uint32_t *rounds_out) {
   TilesEngine *e = c->tiles;
   const RoundCfg R = round_cfg(c, G, n0);
+  if (R.lds > kLdsBytes)
+    return fail(c, CMI_ERR_ARG, "tiles engine: the tile's opacity and accumulator exceed the LDS");
   launch_plan(c, G, R, false);
   hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
                      0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
