@@ -255,13 +255,19 @@ def main():
     ctx.set_sources(pos, L)
     Ltot = ctx.total_luminosity()
     seed = 1
+    # the roofline covers every launch of the dominant kernel in this run
+    # (warm-up and timed transports alike), as a kernel trace of the same
+    # command does
+    ctx.kernel_timing(reset=True)
+    all_cross = all_hops = 0
     for it in range(args.warmup):
         s = ctx.transport(N, seed, it)
         l1 = ctx.update_ionization(Ltot, N, W.ALPHA_B, form=1)
+        all_cross += s["crossings"]
+        all_hops += s["hops"]
         log("warmup %d: %.3f s transport, %d crossings, L1 %.3g"
             % (it, s["seconds"], s["crossings"], l1))
     ctx.synchronize()
-    ctx.kernel_timing(reset=True)
     cross = hops = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -275,8 +281,10 @@ def main():
     t1 = time.perf_counter()
     dt = t1 - t0
     avg_kernel, calls = ctx.kernel_timing()
+    all_cross += cross
+    all_hops += hops
     b_cross = 20 if prec == cmi.PREC_FP32 else 32
-    per_launch = (cross * b_cross + hops * 64) / max(calls, 1)
+    per_launch = (all_cross * b_cross + all_hops * 64) / max(calls, 1)
     achieved = per_launch / avg_kernel
     out = {
         "metric": METRIC,
@@ -316,9 +324,12 @@ def main():
             "kernel": "walk_emit_kernel" if args.engine == "walk"
                       else "tiles_process_kernel",
             "launches": calls,
-            "bytes_model": ("%d B/crossing + 64 B/%s visit, averaged per "
-                            "launch" % (b_cross, "subgrid" if args.engine ==
-                                        "walk" else "tile")),
+            "bytes_model": ("%d B/crossing + 64 B/%s visit, averaged over "
+                            "the launches of the warm-up and timed "
+                            "transports" % (b_cross, "subgrid" if args.engine ==
+                                            "walk" else "tile")),
+            "crossings_per_launch": all_cross / max(calls, 1),
+            "visits_per_launch": all_hops / max(calls, 1),
             "avg_kernel_s": avg_kernel,
         },
     }
@@ -328,6 +339,9 @@ def main():
             prec == cmi.PREC_FP32 and os.path.exists(traffic_file)):
         with open(traffic_file) as f:
             tr = json.load(f)
+    else:
+        tr = None
+    if tr and tr.get("steps") == args.steps and tr.get("warmup") == args.warmup:
         # HBM bytes per launch of the same kernel on the same workload, from
         # the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs
         out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
