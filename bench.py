@@ -172,6 +172,50 @@ def warm_start_section(args, box, ncell, nsub, n_ref, pos, L):
     }
 
 
+def precision_section(args):
+    """The uniform Stromgren sphere at 128^3 (8^3 subgrids of 16^3), 1e8
+    packets per iteration, run two ways from x_n = 1e-6 with the same
+    packets: the all-FP64 GPU walk (the reference's arithmetic) and the FP32
+    tiles engine with FP64 deposits.  Crossings/s of each, and the FP32
+    run's neutral-fraction error against the FP64 one and its ionized radius
+    against the analytic Stromgren radius."""
+    import parallelcmacionize_amd as cmi
+    from parallelcmacionize_amd import workloads as W
+    box = W.box_10pc()
+    nc, ns = [128] * 3, [8] * 3
+    N = int(args.precision_packets)
+    its = args.precision_iterations
+    out = {"workload": ("uniform Stromgren sphere 128^3 (8^3 subgrids of 16^3), "
+                        "n_H 1e8 m^-3, Q 4.26e49 s^-1, %.0e packets per "
+                        "iteration, %d iterations, alpha_B 2.7e-19" % (N, its))}
+    xs = {}
+    for name, prec, eng in (("fp64_walk", cmi.PREC_FP64, cmi.ENGINE_WALK),
+                            ("fp32_tiles", cmi.PREC_FP32, cmi.ENGINE_TILES)):
+        with cmi.Context(box, nc, ns, prec) as ctx:
+            ctx.set_engine(eng)
+            ctx.set_fields(np.full(128 ** 3, W.N_H), np.full(128 ** 3, W.XN0))
+            ctx.set_sources([[0.0, 0.0, 0.0]], [W.Q_STROMGREN])
+            cross, secs = 0, 0.0
+            for it in range(its):
+                st = ctx.transport(N, 7, it)
+                cross += st["crossings"]
+                secs += st["seconds"]
+                ctx.update_ionization(W.Q_STROMGREN, N, W.ALPHA_B, form=1)
+            xs[name] = ctx.get_fields()[1]
+        out[name] = {"crossings_per_s": cross / secs, "transport_seconds": secs,
+                     "crossings": cross}
+        log("precision %s: %.3e crossings/s" % (name, cross / secs))
+    Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)
+    d = np.abs(xs["fp32_tiles"] - xs["fp64_walk"])
+    out["xn_error_fp32_vs_fp64"] = {"mean_abs": float(d.mean()),
+                                    "max_abs": float(d.max()),
+                                    "mean_abs_ionized": float(d[xs["fp64_walk"] < 0.5].mean())}
+    for name in xs:
+        out[name]["ionized_radius_over_analytic"] = (
+            W.ionized_radius(xs[name], box, nc) / Rs)
+    return out
+
+
 def batched_section(args):
     """64 problems of uneven size in one graph-captured problem-set
     iteration: throughput and per-problem completion times."""
@@ -214,6 +258,9 @@ def main():
         help="trained warm-start network for the plain-vs-warm comparison")
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-packets", type=float, default=1e8)
+    ap.add_argument("--precision-iterations", type=int, default=10)
     ap.add_argument("--batched-iterations", type=int, default=5)
     ap.add_argument("--tol", type=float, default=1e-3)
     ap.add_argument("--copy-calibration", type=float, default=0,
@@ -359,6 +406,8 @@ def main():
         out["warm_start"] = warm_start_section(args, box, ncell, nsub, n, pos, L)
     if not args.no_batched:
         out["batched"] = batched_section(args)
+    if not args.no_precision:
+        out["precision_128_uniform"] = precision_section(args)
     print(json.dumps(out))
 
 

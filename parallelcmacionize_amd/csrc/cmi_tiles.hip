@@ -488,6 +488,73 @@ __global__ void __launch_bounds__(kScatterThreads) tiles_scatter_kernel(
   }
 }
 
+// Scatter of one round's survivors, by the round's descriptors.  A
+// descriptor's output region [start, start + count) holds its survivors
+// (destination tiles; kNone after them), and a packet leaves a tile through
+// one of its 6 faces, so a region has at most 6 destinations: a workgroup
+// counts its region's destinations in a small LDS table, reserves each
+// destination's range with one global atomic, then ranks the slots again
+// and writes the index list.  (A destination that finds the table full
+// takes a global atomic per slot.)
+constexpr int kDescScatterThreads = 256;
+constexpr int kDescScatterTable = 8;
+constexpr int kDescScatterBlocks = 4096;
+
+__device__ __forceinline__ int desc_table_slot(uint32_t *keys, uint32_t t, bool insert) {
+#pragma unroll
+  for (int e = 0; e < kDescScatterTable; ++e) {
+    const uint32_t k = keys[e];
+    if (k == t) return e;
+    if (k == kNone) {
+      if (!insert) return -1;
+      const uint32_t old = atomicCAS(&keys[e], kNone, t);
+      if (old == kNone || old == t) return e;
+    }
+  }
+  return -1;
+}
+
+__global__ void __launch_bounds__(kDescScatterThreads) tiles_scatter_desc_kernel(
+    const Desc *__restrict__ desc, const uint32_t *__restrict__ ndesc_ptr,
+    const uint32_t *__restrict__ dest, const uint32_t *__restrict__ seg,
+    uint32_t *__restrict__ cursor, uint32_t *__restrict__ idx) {
+  __shared__ uint32_t keys[kDescScatterTable];
+  __shared__ uint32_t cnts[kDescScatterTable];
+  __shared__ uint32_t bases[kDescScatterTable];
+  const uint32_t nd = *ndesc_ptr;
+  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
+    const Desc D = desc[di];
+    if (threadIdx.x < kDescScatterTable) {
+      keys[threadIdx.x] = kNone;
+      cnts[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < D.count; k += kDescScatterThreads) {
+      const uint32_t t = dest[D.start + k];
+      if (t == kNone) continue;
+      const int e = desc_table_slot(keys, t, true);
+      if (e >= 0) atomicAdd(&cnts[e], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kDescScatterTable) {
+      const uint32_t c = cnts[threadIdx.x];
+      if (c) bases[threadIdx.x] = atomicAdd(&cursor[keys[threadIdx.x]], c);
+      cnts[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < D.count; k += kDescScatterThreads) {
+      const uint32_t s = D.start + k;
+      const uint32_t t = dest[s];
+      if (t == kNone) continue;
+      const int e = desc_table_slot(keys, t, false);
+      const uint32_t b = e >= 0 ? bases[e] + atomicAdd(&cnts[e], 1u)
+                                : atomicAdd(&cursor[t], 1u);
+      idx[seg[t] + b] = s;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Process: one tile visit per descriptor, FP32 DDA inside the tile.
 // ---------------------------------------------------------------------------
@@ -560,6 +627,14 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     }
     int ttx, tty, ttz;
     tile_coords(P, tile, ttx, tty, ttz);
+    // the problem's extent and the 6 face neighbours of the tile in
+    // registers: a scalar load inside the crossing loop would make every LDS
+    // access wait for it (one shared counter), and the neighbours' indices
+    // need integer divisions
+    const int PN0 = P.N[0], PN1 = P.N[1], PN2 = P.N[2];
+    const int nb_xp = tile_index(P, ttx + 1, tty, ttz), nb_xn = tile_index(P, ttx - 1, tty, ttz);
+    const int nb_yp = tile_index(P, ttx, tty + 1, ttz), nb_yn = tile_index(P, ttx, tty - 1, ttz);
+    const int nb_zp = tile_index(P, ttx, tty, ttz + 1), nb_zn = tile_index(P, ttx, tty, ttz - 1);
     const int ox = ttx << G.tshift[0], oy = tty << G.tshift[1],
               oz = ttz << G.tshift[2];
 
@@ -619,14 +694,11 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
           const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
           if (moving) {
-            const int ntx = gx >> G.tshift[0], nty = gy >> G.tshift[1],
-                      ntz = gz >> G.tshift[2];
-            const float sx = (float)((ntx - ttx) * T0), sy = (float)((nty - tty) * T1),
-                        sz = (float)((ntz - ttz) * T2);
+            // the new tile differs along the exit axis only
             Rec32 r;
-            r.a.x = (exit_axis == 0) ? (d0 > 0.f ? 0.f : (float)T0) : q0 - sx;
-            r.a.y = (exit_axis == 1) ? (d1 > 0.f ? 0.f : (float)T1) : q1 - sy;
-            r.a.z = (exit_axis == 2) ? (d2 > 0.f ? 0.f : (float)T2) : q2 - sz;
+            r.a.x = (exit_axis == 0) ? (d0 > 0.f ? 0.f : (float)T0) : q0;
+            r.a.y = (exit_axis == 1) ? (d1 > 0.f ? 0.f : (float)T1) : q1;
+            r.a.z = (exit_axis == 2) ? (d2 > 0.f ? 0.f : (float)T2) : q2;
             r.a.w = d0;
             r.b.x = d1;
             r.b.y = d2;
@@ -634,10 +706,13 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             r.b.w = __uint_as_float(pack_cell(gx, gy, gz));
             const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
             Pout[slot] = r;
-            dest_out[slot] = (uint32_t)tile_index(P, ntx, nty, ntz);
-            if (kList) pid_out[slot] = my_pid;
             const float de = exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2);
-            atomicAdd(&s_face[2 * exit_axis + (de > 0.f ? 0 : 1)], 1u);
+            const bool up = de > 0.f;
+            dest_out[slot] = (uint32_t)(exit_axis == 0 ? (up ? nb_xp : nb_xn)
+                                        : exit_axis == 1 ? (up ? nb_yp : nb_yn)
+                                                         : (up ? nb_zp : nb_zn));
+            if (kList) pid_out[slot] = my_pid;
+            atomicAdd(&s_face[2 * exit_axis + (up ? 0 : 1)], 1u);
           } else if (kList) {
             po.final_a[my_pid] = make_float4(q0, q1, q2, d0);
             po.final_b[my_pid] = make_float4(d1, d2, tau, 0.f);
@@ -717,9 +792,13 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const float kap = kap_c;
       const float dtau = seg * kap;
       ++vis_cross;
-      if (dtau >= tau) {
-        const float sa = kap > 0.f ? __fdividef(tau, kap) : 0.f;
-        AccAtomic<Acc>::add(&accL[off], sa);
+      const bool absorbed = dtau >= tau;
+      // the path deposited in this cell: to its face, or the partial last
+      // step up to the target optical depth
+      const float sa =
+          absorbed ? (kap > 0.f ? tau * __builtin_amdgcn_rcpf(kap) : 0.f) : seg;
+      AccAtomic<Acc>::add(&accL[off], sa);
+      if (absorbed) {
         t = t + sa;
         if (kReemit) {
           const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
@@ -772,7 +851,6 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         continue;
       }
       tau -= dtau;
-      AccAtomic<Acc>::add(&accL[off], seg);
       t = fmaxf(t, tn);
       const bool a0 = tm0 == tn;
       const bool a1 = !a0 && (tm1 == tn);
@@ -786,18 +864,19 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       tm2 += a2 ? td2 : 0.f;
       const bool out0 = (unsigned)l0 >= (unsigned)T0, out1 = (unsigned)l1 >= (unsigned)T1,
                  out2 = (unsigned)l2 >= (unsigned)T2;
-      if (!(out0 | out1 | out2)) kap_c = kapL[off];  // the next cell, read ahead
-      if (out0 | out1 | out2) {
-        exit_axis = out0 ? 0 : (out1 ? 1 : 2);
-        const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
-        const bool in_box = (unsigned)gx < (unsigned)P.N[0] &&
-                            (unsigned)gy < (unsigned)P.N[1] &&
-                            (unsigned)gz < (unsigned)P.N[2];
-        outcome = in_box ? 1 : 2;
-        live = false;
-        pending = true;
-      } else if (++steps > max_steps) {
-        outcome = 3;
+      const bool out = out0 | out1 | out2;
+      ++steps;
+      if (!out) kap_c = kapL[off];  // the next cell, read ahead
+      if (out | (steps > max_steps)) {
+        if (out) {
+          exit_axis = out0 ? 0 : (out1 ? 1 : 2);
+          const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
+          const bool in_box = (unsigned)gx < (unsigned)PN0 && (unsigned)gy < (unsigned)PN1 &&
+                              (unsigned)gz < (unsigned)PN2;
+          outcome = in_box ? 1 : 2;
+        } else {
+          outcome = 3;  // walk bound
+        }
         live = false;
         pending = true;
       }
@@ -816,11 +895,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       if (threadIdx.x < 6) {
         const uint32_t c = s_face[threadIdx.x];
         if (c) {
-          const int ax = threadIdx.x >> 1;
-          const int sg = (threadIdx.x & 1) ? -1 : 1;
-          const int nt = tile_index(P, ttx + (ax == 0 ? sg : 0),
-                                    tty + (ax == 1 ? sg : 0),
-                                    ttz + (ax == 2 ? sg : 0));
+          const int f = threadIdx.x;
+          const int nt = f == 0 ? nb_xp : f == 1 ? nb_xn : f == 2 ? nb_yp
+                       : f == 3 ? nb_yn : f == 4 ? nb_zp : nb_zn;
           atomicAdd(&hist_next[nt], c);
         }
       }
@@ -950,8 +1027,11 @@ struct TilesEngine {
   uint32_t *pid[2] = {nullptr, nullptr};
   uint32_t *idx = nullptr;
   uint32_t *hist = nullptr, *seg = nullptr, *cursor = nullptr;
-  Desc *desc = nullptr;
-  uint32_t *ndesc = nullptr;
+  // descriptors of the round being processed and of the next one (the
+  // scatter of a round's survivors reads that round's descriptors after the
+  // next round is planned)
+  Desc *desc = nullptr, *desc2 = nullptr;
+  uint32_t *ndesc = nullptr, *ndesc2 = nullptr;
   uint32_t *total = nullptr;   // [2]: packets of the round being planned
   unsigned *ticket = nullptr;
   uint32_t *h_total = nullptr;  // pinned
@@ -988,6 +1068,8 @@ static void free_engine(TilesEngine *e) {
   hipFree(e->cursor);
   hipFree(e->desc);
   hipFree(e->ndesc);
+  hipFree(e->desc2);
+  hipFree(e->ndesc2);
   hipFree(e->total);
   hipFree(e->ticket);
   if (e->h_total) hipHostFree(e->h_total);
@@ -1098,6 +1180,8 @@ static int ensure_engine(cmi_ctx *c, const std::vector<ProbGeo> &probs,
   if (err == hipSuccess) err = hipMalloc(&e->cursor, ntiles * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->desc, e->desc_cap * sizeof(Desc));
   if (err == hipSuccess) err = hipMalloc(&e->ndesc, sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->desc2, e->desc_cap * sizeof(Desc));
+  if (err == hipSuccess) err = hipMalloc(&e->ndesc2, sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->total, 2 * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->ticket, sizeof(unsigned));
   if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 2 * sizeof(uint32_t));
@@ -1209,11 +1293,14 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
 #undef CMI_PROC
 }
 
+// Plan into the current descriptors (next = false: before round 0) or into
+// the next round's (next = true: after a process launch).
 static void launch_plan(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
-                        bool track) {
+                        bool track, bool next = false) {
   TilesEngine *e = c->tiles;
   hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
-                     G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
+                     G.ntiles, e->hist, e->seg, e->cursor, next ? e->desc2 : e->desc,
+                     next ? e->ndesc2 : e->ndesc,
                      e->total, e->ticket, R.chunk, e->d_tile_prob, e->nprob,
                      track ? e->d_prob_round : nullptr, e->d_prob_time,
                      e->d_round_ctr);
@@ -1246,7 +1333,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
     // total[0] = packets processed this round = slots written to P[1];
     // keep it in total[1] for the scatter, then plan the next round
     hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, c->stream, e->total);
-    launch_plan(c, G, R, false);
+    launch_plan(c, G, R, false, true);
     CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream));
     CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
@@ -1254,10 +1341,12 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
       ++r;
       break;
     }
-    hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
-                       0, c->stream, e->total + 1, e->dest[1], e->seg, e->cursor,
-                       e->idx);
+    hipLaunchKernelGGL(tiles_scatter_desc_kernel, dim3(kDescScatterBlocks),
+                       dim3(kDescScatterThreads), 0, c->stream, e->desc, e->ndesc,
+                       e->dest[1], e->seg, e->cursor, e->idx);
     CMI_TRY_HIP(c, hipGetLastError());
+    std::swap(e->desc, e->desc2);
+    std::swap(e->ndesc, e->ndesc2);
     std::swap(e->P[0], e->P[1]);
     std::swap(e->dest[0], e->dest[1]);
     if (with_pid) std::swap(e->pid[0], e->pid[1]);
