@@ -656,6 +656,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     int outcome = 0;  // 0 absorbed, 1 moved to a neighbour tile, 2 escaped, 3 error
     int l0 = 0, l1 = 0, l2 = 0, off = 0, steps = 0, exit_axis = 0;
     int so0 = 0, so1 = 0, so2 = 0;  // signed LDS strides along the direction
+    int sg0 = 0, sg1 = 0, sg2 = 0;  // index steps along the direction
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
     float tm0 = 0.f, tm1 = 0.f, tm2 = 0.f, td0 = 0.f, td1 = 0.f, td2 = 0.f;
     float t = 0.f, tau = 0.f;
@@ -761,6 +762,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             so0 = d0 > 0.f ? st0 : -st0;
             so1 = d1 > 0.f ? st1 : -st1;
             so2 = d2 > 0.f ? 1 : -1;
+            sg0 = d0 > 0.f ? 1 : -1;
+            sg1 = d1 > 0.f ? 1 : -1;
+            sg2 = so2;
             off = (l0 * T1 + l1) * T2 + l2;
             kap_c = kapL[off];
             live = true;
@@ -842,6 +846,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             so0 = d0 > 0.f ? st0 : -st0;
             so1 = d1 > 0.f ? st1 : -st1;
             so2 = d2 > 0.f ? 1 : -1;
+            sg0 = d0 > 0.f ? 1 : -1;
+            sg1 = d1 > 0.f ? 1 : -1;
+            sg2 = so2;
             continue;
           }
         }
@@ -855,9 +862,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const bool a0 = tm0 == tn;
       const bool a1 = !a0 && (tm1 == tn);
       const bool a2 = !a0 && !a1;
-      l0 += a0 ? (d0 > 0.f ? 1 : -1) : 0;
-      l1 += a1 ? (d1 > 0.f ? 1 : -1) : 0;
-      l2 += a2 ? (d2 > 0.f ? 1 : -1) : 0;
+      l0 += a0 ? sg0 : 0;
+      l1 += a1 ? sg1 : 0;
+      l2 += a2 ? sg2 : 0;
       off += a0 ? so0 : (a1 ? so1 : so2);
       tm0 += a0 ? td0 : 0.f;
       tm1 += a1 ? td1 : 0.f;
@@ -865,9 +872,12 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const bool out0 = (unsigned)l0 >= (unsigned)T0, out1 = (unsigned)l1 >= (unsigned)T1,
                  out2 = (unsigned)l2 >= (unsigned)T2;
       const bool out = out0 | out1 | out2;
-      ++steps;
+      // walk bound: crossings of this visit (of this flight after a
+      // reemission)
+      if (kReemit) ++steps;
+      const int nsteps = kReemit ? steps : (int)vis_cross;
       if (!out) kap_c = kapL[off];  // the next cell, read ahead
-      if (out | (steps > max_steps)) {
+      if (out | (nsteps > max_steps)) {
         if (out) {
           exit_axis = out0 ? 0 : (out1 ? 1 : 2);
           const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
