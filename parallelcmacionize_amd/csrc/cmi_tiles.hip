@@ -488,70 +488,21 @@ __global__ void __launch_bounds__(kScatterThreads) tiles_scatter_kernel(
   }
 }
 
-// Scatter of one round's survivors, by the round's descriptors.  A
-// descriptor's output region [start, start + count) holds its survivors
-// (destination tiles; kNone after them), and a packet leaves a tile through
-// one of its 6 faces, so a region has at most 6 destinations: a workgroup
-// counts its region's destinations in a small LDS table, reserves each
-// destination's range with one global atomic, then ranks the slots again
-// and writes the index list.  (A destination that finds the table full
-// takes a global atomic per slot.)
-constexpr int kDescScatterThreads = 256;
-constexpr int kDescScatterTable = 8;
-constexpr int kDescScatterBlocks = 4096;
-
-__device__ __forceinline__ int desc_table_slot(uint32_t *keys, uint32_t t, bool insert) {
-#pragma unroll
-  for (int e = 0; e < kDescScatterTable; ++e) {
-    const uint32_t k = keys[e];
-    if (k == t) return e;
-    if (k == kNone) {
-      if (!insert) return -1;
-      const uint32_t old = atomicCAS(&keys[e], kNone, t);
-      if (old == kNone || old == t) return e;
-    }
-  }
-  return -1;
-}
-
-__global__ void __launch_bounds__(kDescScatterThreads) tiles_scatter_desc_kernel(
-    const Desc *__restrict__ desc, const uint32_t *__restrict__ ndesc_ptr,
-    const uint32_t *__restrict__ dest, const uint32_t *__restrict__ seg,
-    uint32_t *__restrict__ cursor, uint32_t *__restrict__ idx) {
-  __shared__ uint32_t keys[kDescScatterTable];
-  __shared__ uint32_t cnts[kDescScatterTable];
-  __shared__ uint32_t bases[kDescScatterTable];
-  const uint32_t nd = *ndesc_ptr;
-  for (uint32_t di = blockIdx.x; di < nd; di += gridDim.x) {
-    const Desc D = desc[di];
-    if (threadIdx.x < kDescScatterTable) {
-      keys[threadIdx.x] = kNone;
-      cnts[threadIdx.x] = 0;
-    }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < D.count; k += kDescScatterThreads) {
-      const uint32_t t = dest[D.start + k];
-      if (t == kNone) continue;
-      const int e = desc_table_slot(keys, t, true);
-      if (e >= 0) atomicAdd(&cnts[e], 1u);
-    }
-    __syncthreads();
-    if (threadIdx.x < kDescScatterTable) {
-      const uint32_t c = cnts[threadIdx.x];
-      if (c) bases[threadIdx.x] = atomicAdd(&cursor[keys[threadIdx.x]], c);
-      cnts[threadIdx.x] = 0;
-    }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < D.count; k += kDescScatterThreads) {
-      const uint32_t s = D.start + k;
-      const uint32_t t = dest[s];
-      if (t == kNone) continue;
-      const int e = desc_table_slot(keys, t, false);
-      const uint32_t b = e >= 0 ? bases[e] + atomicAdd(&cnts[e], 1u)
-                                : atomicAdd(&cursor[t], 1u);
-      idx[seg[t] + b] = s;
-    }
-    __syncthreads();
+// Placement of one round's survivors: the process kernel left each survivor
+// slot its position relative to its destination tile's segment (the visit's
+// base for that face, reserved by the histogram atomic, plus the survivor's
+// rank among the visit's exits through that face), so the index list is one
+// streaming pass without atomics.
+__global__ void tiles_place_kernel(const uint32_t *__restrict__ nslots_ptr,
+                                   const uint32_t *__restrict__ dest,
+                                   const uint32_t *__restrict__ rel,
+                                   const uint32_t *__restrict__ seg,
+                                   uint32_t *__restrict__ idx) {
+  const uint32_t n = *nslots_ptr;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n;
+       s += gridDim.x * blockDim.x) {
+    const uint32_t t = dest[s];
+    if (t != kNone) idx[seg[t] + rel[s]] = s;
   }
 }
 
@@ -584,12 +535,14 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     uint32_t *__restrict__ dest_out, uint32_t *__restrict__ hist_next,
     const uint32_t *__restrict__ pid_in, uint32_t *__restrict__ pid_out,
     PacketOut po, Counters *__restrict__ cnt,
-    unsigned long long *__restrict__ prob_cross, int refill_batch) {
+    unsigned long long *__restrict__ prob_cross, int refill_batch,
+    uint32_t *__restrict__ rel_out) {
   extern __shared__ __align__(16) unsigned char smem[];
   float *kapL = reinterpret_cast<float *>(smem);
   Acc *accL = reinterpret_cast<Acc *>(smem + sizeof(float) * G.tile_cells);
   __shared__ uint32_t s_next, s_surv, s_desc;
   __shared__ uint32_t s_face[6];
+  __shared__ uint32_t s_fbase[6];
 
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -713,7 +666,10 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
                                         : exit_axis == 1 ? (up ? nb_yp : nb_yn)
                                                          : (up ? nb_zp : nb_zn));
             if (kList) pid_out[slot] = my_pid;
-            atomicAdd(&s_face[2 * exit_axis + (up ? 0 : 1)], 1u);
+            // the rank among this visit's exits through this face; the
+            // face's base is added once the visit has ended
+            const uint32_t f = 2u * (uint32_t)exit_axis + (up ? 0u : 1u);
+            rel_out[slot] = (f << 29) | atomicAdd(&s_face[f], 1u);
           } else if (kList) {
             po.final_a[my_pid] = make_float4(q0, q1, q2, d0);
             po.final_b[my_pid] = make_float4(d1, d2, tau, 0.f);
@@ -903,13 +859,22 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       for (uint32_t i = ns + threadIdx.x; i < D.count; i += kThreads)
         dest_out[D.start + i] = kNone;
       if (threadIdx.x < 6) {
+        // the next round's count of each neighbour; the old value is this
+        // visit's base within the neighbour's segment
         const uint32_t c = s_face[threadIdx.x];
+        uint32_t base = 0;
         if (c) {
           const int f = threadIdx.x;
           const int nt = f == 0 ? nb_xp : f == 1 ? nb_xn : f == 2 ? nb_yp
                        : f == 3 ? nb_yn : f == 4 ? nb_zp : nb_zn;
-          atomicAdd(&hist_next[nt], c);
+          base = atomicAdd(&hist_next[nt], c);
         }
+        s_fbase[threadIdx.x] = base;
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < ns; i += kThreads) {
+        const uint32_t v = rel_out[D.start + i];
+        rel_out[D.start + i] = s_fbase[v >> 29] + (v & ((1u << 29) - 1u));
       }
       if (prob_cross) {
         // crossings of this visit, per problem (one atomic per wave)
@@ -1036,12 +1001,10 @@ struct TilesEngine {
   uint32_t *dest[2] = {nullptr, nullptr};
   uint32_t *pid[2] = {nullptr, nullptr};
   uint32_t *idx = nullptr;
+  uint32_t *rel = nullptr;      // survivors' positions in their next segment
   uint32_t *hist = nullptr, *seg = nullptr, *cursor = nullptr;
-  // descriptors of the round being processed and of the next one (the
-  // scatter of a round's survivors reads that round's descriptors after the
-  // next round is planned)
-  Desc *desc = nullptr, *desc2 = nullptr;
-  uint32_t *ndesc = nullptr, *ndesc2 = nullptr;
+  Desc *desc = nullptr;        // work descriptors of the round
+  uint32_t *ndesc = nullptr;
   uint32_t *total = nullptr;   // [2]: packets of the round being planned
   unsigned *ticket = nullptr;
   uint32_t *h_total = nullptr;  // pinned
@@ -1073,13 +1036,12 @@ static void free_engine(TilesEngine *e) {
     hipFree(e->pid[i]);
   }
   hipFree(e->idx);
+  hipFree(e->rel);
   hipFree(e->hist);
   hipFree(e->seg);
   hipFree(e->cursor);
   hipFree(e->desc);
   hipFree(e->ndesc);
-  hipFree(e->desc2);
-  hipFree(e->ndesc2);
   hipFree(e->total);
   hipFree(e->ticket);
   if (e->h_total) hipHostFree(e->h_total);
@@ -1185,13 +1147,12 @@ static int ensure_engine(cmi_ctx *c, const std::vector<ProbGeo> &probs,
     if (err == hipSuccess && with_pid) err = hipMalloc(&e->pid[i], cap * sizeof(uint32_t));
   }
   if (err == hipSuccess) err = hipMalloc(&e->idx, cap * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->rel, cap * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->hist, ntiles * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->seg, ntiles * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->cursor, ntiles * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->desc, e->desc_cap * sizeof(Desc));
   if (err == hipSuccess) err = hipMalloc(&e->ndesc, sizeof(uint32_t));
-  if (err == hipSuccess) err = hipMalloc(&e->desc2, e->desc_cap * sizeof(Desc));
-  if (err == hipSuccess) err = hipMalloc(&e->ndesc2, sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->total, 2 * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->ticket, sizeof(unsigned));
   if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 2 * sizeof(uint32_t));
@@ -1245,6 +1206,7 @@ struct RoundCfg {
   bool fp32acc;
   size_t lds;
   unsigned scatter_blocks;
+  unsigned place_blocks;
 };
 
 static RoundCfg round_cfg(const cmi_ctx *c, const TileGeo &G, size_t n0) {
@@ -1260,6 +1222,7 @@ static RoundCfg round_cfg(const cmi_ctx *c, const TileGeo &G, size_t n0) {
   r.grid = c->num_cus * wg_per_cu;
   const size_t per_block = (size_t)kScatterThreads * kScatterItems;
   r.scatter_blocks = (unsigned)std::min<size_t>(8192, (n0 + per_block - 1) / per_block + 1);
+  r.place_blocks = (unsigned)std::min<size_t>(8192, (n0 + 4 * 256 - 1) / (4 * 256) + 1);
   return r;
 }
 
@@ -1272,19 +1235,19 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
                      R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc, \
                      e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,   \
                      LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po, \
-                     c->d_cnt, prob_cross, R.refill)
+                     c->d_cnt, prob_cross, R.refill, e->rel)
   if (G.reemit_p > 0.f) {
     // reemission: FP64 LDS accumulators, emission-driven transports only
     if (R.threads == 256)
       hipLaunchKernelGGL((tiles_process_kernel<double, 256, false, true>), dim3(R.grid),
                          dim3(256), R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,
                          e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,
-                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill);
+                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill, e->rel);
     else
       hipLaunchKernelGGL((tiles_process_kernel<double, 512, false, true>), dim3(R.grid),
                          dim3(512), R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,
                          e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,
-                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill);
+                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill, e->rel);
     return;
   }
   if (with_pid) {
@@ -1303,14 +1266,11 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
 #undef CMI_PROC
 }
 
-// Plan into the current descriptors (next = false: before round 0) or into
-// the next round's (next = true: after a process launch).
 static void launch_plan(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
-                        bool track, bool next = false) {
+                        bool track) {
   TilesEngine *e = c->tiles;
   hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
-                     G.ntiles, e->hist, e->seg, e->cursor, next ? e->desc2 : e->desc,
-                     next ? e->ndesc2 : e->ndesc,
+                     G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
                      e->total, e->ticket, R.chunk, e->d_tile_prob, e->nprob,
                      track ? e->d_prob_round : nullptr, e->d_prob_time,
                      e->d_round_ctr);
@@ -1343,7 +1303,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
     // total[0] = packets processed this round = slots written to P[1];
     // keep it in total[1] for the scatter, then plan the next round
     hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, c->stream, e->total);
-    launch_plan(c, G, R, false, true);
+    launch_plan(c, G, R, false);
     CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream));
     CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
@@ -1351,12 +1311,9 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
       ++r;
       break;
     }
-    hipLaunchKernelGGL(tiles_scatter_desc_kernel, dim3(kDescScatterBlocks),
-                       dim3(kDescScatterThreads), 0, c->stream, e->desc, e->ndesc,
-                       e->dest[1], e->seg, e->cursor, e->idx);
+    hipLaunchKernelGGL(tiles_place_kernel, dim3(R.place_blocks), dim3(256), 0, c->stream,
+                       e->total + 1, e->dest[1], e->rel, e->seg, e->idx);
     CMI_TRY_HIP(c, hipGetLastError());
-    std::swap(e->desc, e->desc2);
-    std::swap(e->ndesc, e->ndesc2);
     std::swap(e->P[0], e->P[1]);
     std::swap(e->dest[0], e->dest[1]);
     if (with_pid) std::swap(e->pid[0], e->pid[1]);
