@@ -13,14 +13,17 @@
 //             leaves the tile or is absorbed, the path lengths accumulate in an
 //             LDS copy of the tile's J that is flushed to HBM once per visit;
 //             survivors are written to the workgroup's own output region
-//             (LDS-counter compaction) with their destination tile, and the
-//             per-face exit counts go to the next round's tile histogram;
+//             (LDS-counter compaction) with their destination tile; the
+//             per-face exit counts go to the next round's tile histogram
+//             (whose old values are the visit's bases) and every survivor
+//             gets its position relative to its destination's segment;
 //   plan      one workgroup: exclusive scan of the histogram into tile
 //             segments, and the next round's work descriptors (tile, start,
 //             count) in chunks;
-//   scatter   destination tile of every survivor -> its slot in the tile's
-//             segment (wave-aggregated atomics on a per-tile cursor): the
-//             index list the next round gathers packets through.
+//   place     segment start + relative position of every survivor -> the
+//             index list the next round gathers packets through (the
+//             emission round, which has no visits, is scattered with per-block
+//             LDS counts and per-tile cursors instead).
 //
 // Refills are batched per wave: finished lanes wait until a threshold of
 // lanes is free (or the wave runs dry) and then write out and reload
@@ -412,7 +415,7 @@ __global__ void tiles_begin_kernel(int nprob, uint32_t *__restrict__ prob_round,
   }
 }
 
-// total[1] = total[0] (kept for the scatter), as a kernel so that the round
+// total[1] = total[0] (kept for the placement), as a kernel so that the round
 // loop can be captured in a graph without memcpy nodes.
 __global__ void copy_total_kernel(uint32_t *__restrict__ total) {
   if (threadIdx.x == 0) total[1] = total[0];
@@ -1276,7 +1279,7 @@ static void launch_plan(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
                      e->d_round_ctr);
 }
 
-// Rounds of process / plan / scatter until no packet is left (host-checked
+// Rounds of process / plan / place until no packet is left (host-checked
 // after every round).  Round 0's input (records, destinations, histogram)
 // is already in P[0], dest[0], hist; n0 packets.
 static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
@@ -1301,7 +1304,7 @@ static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
     if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r + 1], c->stream));
     CMI_TRY_HIP(c, hipGetLastError());
     // total[0] = packets processed this round = slots written to P[1];
-    // keep it in total[1] for the scatter, then plan the next round
+    // keep it in total[1] for the placement, then plan the next round
     hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, c->stream, e->total);
     launch_plan(c, G, R, false);
     CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
