@@ -230,6 +230,13 @@ int cmi_get_sizes(const cmi_ctx *ctx, uint64_t *nsubgrid, uint64_t *ncell);
 int cmi_kernel_timing(cmi_ctx *ctx, double *avg_seconds, uint64_t *calls,
                       int reset);
 
+/* Switch the tiles engine's per-launch timing of tiles_process_kernel on
+ * (default) or off.  The two events per round cost about 6 us each on the
+ * stream; runs that do not need cmi_kernel_timing (convergence runs, the
+ * warm start's cheap passes) switch them off.  No counterpart in the
+ * reference, which has no device timing. */
+int cmi_set_kernel_timing(cmi_ctx *ctx, int on);
+
 /* Synchronise the context's stream. */
 int cmi_synchronize(cmi_ctx *ctx);
 

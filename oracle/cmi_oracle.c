@@ -502,7 +502,10 @@ double orc_uniform_open(uint32_t hi, uint32_t lo) {
  * 1650-1677): every source gets floor(L_i / L_tot * N) packets; the
  * remainder goes, one each, to the sources with the smallest random keys.
  * The keys come from Philox (counter {i, 0, iteration, 0x5EED}) instead of
- * the reference's RANLUX stream. */
+ * the reference's RANLUX stream.  Beyond the reference: a rounded share that
+ * overshoots is taken back from the largest share, and a remainder of more
+ * than one packet per source goes round the key order again, so that the
+ * split always sums to N. */
 void orc_source_counts(const double *L, int nsrc, uint64_t npackets,
                        uint64_t seed, uint32_t iteration, uint64_t *counts) {
   double Ltot = 0.;
@@ -511,6 +514,13 @@ void orc_source_counts(const double *L, int nsrc, uint64_t npackets,
   for (int i = 0; i < nsrc; ++i) {
     counts[i] = (uint64_t)(L[i] / Ltot * (double)npackets);
     added += counts[i];
+  }
+  while (added > npackets) {
+    int imax = 0;
+    for (int i = 1; i < nsrc; ++i)
+      if (counts[i] > counts[imax]) imax = i;
+    --counts[imax];
+    --added;
   }
   uint64_t rem = npackets - added;
   if (rem == 0) return;
@@ -534,7 +544,7 @@ void orc_source_counts(const double *L, int nsrc, uint64_t npackets,
       --j;
     }
   }
-  for (uint64_t r = 0; r < rem && r < (uint64_t)nsrc; ++r) counts[order[r]]++;
+  for (uint64_t r = 0; r < rem; ++r) counts[order[r % (uint64_t)nsrc]]++;
   free(keys);
   free(order);
 }

@@ -426,7 +426,7 @@ int cmi_get_fields(cmi_ctx *c, double *J, double *xn) {
 int cmi_reset_intensity(cmi_ctx *c) {
   if (!c) return CMI_ERR_ARG;
   (void)hipSetDevice(c->device);
-  CMI_TRY_HIP(c, launch_scale(c->f.J, c->ncell_total, 0., c->stream));
+  CMI_TRY_HIP(c, launch_fill(c->f.J, c->ncell_total, 0., c->stream));
   CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
   return CMI_OK;
 }
@@ -692,6 +692,12 @@ int cmi_kernel_timing(cmi_ctx *c, double *avg, uint64_t *calls, int reset) {
     c->kernel_time_sum = 0.;
     c->kernel_calls = 0;
   }
+  return CMI_OK;
+}
+
+int cmi_set_kernel_timing(cmi_ctx *c, int on) {
+  if (!c) return CMI_ERR_ARG;
+  c->kernel_timing = on ? 1 : 0;
   return CMI_OK;
 }
 

@@ -41,6 +41,7 @@ struct cmi_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double kernel_time_sum = 0.;
   uint64_t kernel_calls = 0;
+  int kernel_timing = 1;  // record the tiles engine's per-round process events
   cmi::TilesEngine *tiles = nullptr;
   std::string err;
 };
@@ -76,6 +77,16 @@ inline void source_counts(const std::vector<double> &L, uint64_t npackets,
     counts[i] = (uint64_t)(L[i] / Ltot * (double)npackets);
     added += counts[i];
   }
+  // the rounded shares can overshoot by a packet here and there: take the
+  // excess from the largest shares, so that the split always sums to
+  // npackets (the engines index packets by it)
+  while (added > npackets) {
+    int imax = 0;
+    for (int i = 1; i < nsrc; ++i)
+      if (counts[i] > counts[imax]) imax = i;
+    --counts[imax];
+    --added;
+  }
   const uint64_t rem = npackets - added;
   if (rem == 0) return;
   std::vector<double> key(nsrc);
@@ -88,7 +99,7 @@ inline void source_counts(const std::vector<double> &L, uint64_t npackets,
   }
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return key[a] < key[b]; });
-  for (uint64_t r = 0; r < rem && r < (uint64_t)nsrc; ++r) counts[order[r]]++;
+  for (uint64_t r = 0; r < rem; ++r) counts[order[r % (uint64_t)nsrc]]++;
 }
 
 // Tiles engine (cmi_tiles.hip).

@@ -52,8 +52,7 @@ hipError_t launch_ionize(const FieldsDev &f, size_t ncell, double jfac,
 // kappa = kappa_scale * n * x_n for every cell.
 hipError_t launch_kappa(const FieldsDev &f, size_t ncell, double kappa_scale,
                         hipStream_t st);
-// J *= scale for every cell (FP32-mode J -> reference units).
-hipError_t launch_scale(double *J, size_t ncell, double scale,
-                        hipStream_t st);
+// p[i] = value for i < n.
+hipError_t launch_fill(double *p, size_t n, double value, hipStream_t st);
 
 }  // namespace cmi

@@ -68,13 +68,12 @@ __global__ void kappa_kernel(const double *__restrict__ n,
   }
 }
 
-__global__ void scale_kernel(double *__restrict__ J, size_t ncell,
-                             double scale) {
-#pragma clang fp contract(off)
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < ncell;
-       i += (size_t)gridDim.x * blockDim.x) {
-    J[i] = J[i] * scale;
-  }
+// A store, not a scaling: freshly allocated memory may hold NaN or Inf bit
+// patterns, which a multiplication by zero would keep.
+__global__ void fill_kernel(double *__restrict__ p, size_t n, double value) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = value;
 }
 
 hipError_t launch_ionize(const FieldsDev &f, size_t ncell, double jfac,
@@ -96,12 +95,11 @@ hipError_t launch_kappa(const FieldsDev &f, size_t ncell, double kappa_scale,
   return hipGetLastError();
 }
 
-hipError_t launch_scale(double *J, size_t ncell, double scale,
-                        hipStream_t st) {
+hipError_t launch_fill(double *p, size_t n, double value, hipStream_t st) {
   const unsigned blocks =
-      (unsigned)((ncell + 255) / 256 < 4096 ? (ncell + 255) / 256 : 4096);
-  hipLaunchKernelGGL(scale_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, st,
-                     J, ncell, scale);
+      (unsigned)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, st,
+                     p, n, value);
   return hipGetLastError();
 }
 

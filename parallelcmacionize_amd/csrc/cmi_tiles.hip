@@ -157,10 +157,13 @@ struct PacketOut {
 // ---------------------------------------------------------------------------
 // Packet k of problem p draws counters {k_lo, k_hi ^ (p << 20), iteration, 0}
 // and {..., 1} (problem 0 draws exactly what emit_packet() draws).
+// Returns the problem; *src is the emitting source and *rank the packet's
+// number among that source's packets.
 __device__ __forceinline__ int set_emit(const SetSources &S, uint64_t kg,
                                         uint32_t iteration, uint32_t key0,
                                         uint32_t key1, double pos[3],
-                                        double dir[3], double *tau) {
+                                        double dir[3], double *tau, int *src,
+                                        uint64_t *rank) {
   int p = 0;
   while (p + 1 < S.nprob && kg >= S.pk_cum[p]) ++p;
   const uint64_t k = kg - (p ? S.pk_cum[p - 1] : 0);
@@ -185,19 +188,28 @@ __device__ __forceinline__ int set_emit(const SetSources &S, uint64_t kg,
   pos[0] = S.pos[s][0];
   pos[1] = S.pos[s][1];
   pos[2] = S.pos[s][2];
+  *src = s;
+  *rank = k - (s > S.src_begin[p] ? S.src_cum[s - 1] : 0);
   return p;
 }
 
 // Seed and iteration come from S (device memory), so that a captured graph
-// of the iteration can be replayed with new values.
+// of the iteration can be replayed with new values.  Round 0 is planned
+// before emission (the per-tile counts follow from the source split), so
+// every packet also goes straight into round 0's index list: its source's
+// share of the tile's segment starts at src_off[source].
 __global__ void tiles_emit_kernel(TileGeo G, const SetSources *__restrict__ S,
                                   uint64_t npackets, Rec32 *__restrict__ out,
-                                  uint32_t *__restrict__ dest) {
+                                  const uint32_t *__restrict__ seg,
+                                  const uint32_t *__restrict__ src_off,
+                                  uint32_t *__restrict__ idx) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < npackets) {
     double pos[3], dir[3], tau;
+    int src;
+    uint64_t rank;
     const int p = set_emit(*S, k, S->iteration, (uint32_t)S->seed,
-                           (uint32_t)(S->seed >> 32), pos, dir, &tau);
+                           (uint32_t)(S->seed >> 32), pos, dir, &tau, &src, &rank);
     const ProbGeo &P = G.probs[p];
     int g[3];
     double pc[3];
@@ -220,15 +232,20 @@ __global__ void tiles_emit_kernel(TileGeo G, const SetSources *__restrict__ S,
     r.b.z = (float)tau;
     r.b.w = __uint_as_float(pack_cell(g[0], g[1], g[2]));
     out[k] = r;
-    dest[k] = tile;
+    // in range whenever the source split sums to npackets (as it does by
+    // construction); the check keeps a broken split from writing past idx
+    const uint64_t at = (uint64_t)seg[tile] + src_off[src] + rank;
+    if (at < npackets) idx[at] = (uint32_t)k;
   }
 }
 
 // Tile of every source's start cell (the cell the emission kernel files its
-// packets under) gets that source's packet count.
+// packets under) gets that source's packet count; the old count is the
+// source's offset within the tile's round-0 segment.
 __global__ void tiles_source_hist_kernel(TileGeo G,
                                          const SetSources *__restrict__ S,
-                                         uint32_t *__restrict__ hist) {
+                                         uint32_t *__restrict__ hist,
+                                         uint32_t *__restrict__ src_off) {
   for (int p = 0; p < S->nprob; ++p) {
     const ProbGeo &P = G.probs[p];
     const int b = S->src_begin[p], e = S->src_begin[p + 1];
@@ -244,7 +261,7 @@ __global__ void tiles_source_hist_kernel(TileGeo G,
       }
       const int tile = tile_index(P, g[0] >> G.tshift[0], g[1] >> G.tshift[1],
                                   g[2] >> G.tshift[2]);
-      atomicAdd(&hist[tile], (uint32_t)c);
+      src_off[s] = atomicAdd(&hist[tile], (uint32_t)c);
     }
   }
 }
@@ -330,6 +347,15 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v,
   return r;
 }
 
+// A tile with more than kPlanSerialChunks chunks has its descriptors written
+// by the whole workgroup (a source's tile holds millions of packets in the
+// first rounds); up to kPlanCrowded such tiles per plan, the rest serially.
+constexpr uint32_t kPlanSerialChunks = 4;
+constexpr int kPlanCrowded = 512;
+
+// total[0]: packets of the round being planned (in: those of the round just
+// processed, kept in total[1] for the placement); nonempty (optional) counts
+// the plans that found packets.
 __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     int ntiles, uint32_t *__restrict__ hist, uint32_t *__restrict__ seg,
     uint32_t *__restrict__ cursor, Desc *__restrict__ desc,
@@ -338,15 +364,20 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     const uint16_t *__restrict__ tile_prob, int nprob,
     uint32_t *__restrict__ prob_round,
     unsigned long long *__restrict__ prob_time,
-    uint32_t *__restrict__ round_ctr) {
+    uint32_t *__restrict__ round_ctr, uint32_t *__restrict__ nonempty) {
   __shared__ uint64_t sh[17];
   __shared__ uint32_t s_pc[kMaxProblems];
   __shared__ uint32_t s_round;
+  __shared__ uint32_t s_ncrowded;
+  __shared__ uint4 s_crowded[kPlanCrowded];  // tile, first descriptor, count, segment start
   const bool track = prob_round != nullptr;
-  if (track) {
-    for (int q = threadIdx.x; q < nprob; q += kPlanThreads) s_pc[q] = 0;
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    total[1] = total[0];
+    s_ncrowded = 0;
   }
+  if (track)
+    for (int q = threadIdx.x; q < nprob; q += kPlanThreads) s_pc[q] = 0;
+  __syncthreads();
   const int per = (ntiles + kPlanThreads - 1) / kPlanThreads;
   const int t0 = threadIdx.x * per;
   const int t1 = min(ntiles, t0 + per);
@@ -365,22 +396,43 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     seg[t] = (uint32_t)base_c;
     cursor[t] = 0;
     hist[t] = 0;
-    if (c) {
-      for (uint32_t s = 0; s < c; s += chunk) {
+    const uint32_t nch = (c + chunk - 1) / chunk;
+    uint32_t slot = kPlanCrowded;
+    if (nch > kPlanSerialChunks) slot = atomicAdd(&s_ncrowded, 1u);
+    if (slot < kPlanCrowded) {
+      s_crowded[slot] = make_uint4((uint32_t)t, (uint32_t)base_d, c, (uint32_t)base_c);
+    } else {
+      for (uint32_t j = 0; j < nch; ++j) {
         Desc d;
         d.tile = t;
-        d.start = (uint32_t)base_c + s;
-        d.count = (c - s < chunk) ? (c - s) : chunk;
+        d.start = (uint32_t)base_c + j * chunk;
+        d.count = (c - j * chunk < chunk) ? (c - j * chunk) : chunk;
         d.pad = 0;
-        desc[base_d++] = d;
+        desc[base_d + j] = d;
       }
     }
+    base_d += nch;
     base_c += c;
+  }
+  __syncthreads();
+  const uint32_t ncrowded = min(s_ncrowded, (uint32_t)kPlanCrowded);
+  for (uint32_t b = 0; b < ncrowded; ++b) {
+    const uint4 B = s_crowded[b];
+    const uint32_t nch = (B.z + chunk - 1) / chunk;
+    for (uint32_t j = threadIdx.x; j < nch; j += kPlanThreads) {
+      Desc d;
+      d.tile = (int)B.x;
+      d.start = B.w + j * chunk;
+      d.count = (B.z - j * chunk < chunk) ? (B.z - j * chunk) : chunk;
+      d.pad = 0;
+      desc[B.y + j] = d;
+    }
   }
   if (threadIdx.x == 0) {
     *ndesc = (uint32_t)tot_d;
-    *total = (uint32_t)tot_c;
+    total[0] = (uint32_t)tot_c;
     *ticket = 0;
+    if (nonempty && tot_c) *nonempty += 1u;
   }
   if (track) {
     // completion of each problem: the first plan that finds it empty
@@ -413,12 +465,6 @@ __global__ void tiles_begin_kernel(int nprob, uint32_t *__restrict__ prob_round,
     *round_ctr = 0;
     *t0 = wall_clock64();
   }
-}
-
-// total[1] = total[0] (kept for the placement), as a kernel so that the round
-// loop can be captured in a graph without memcpy nodes.
-__global__ void copy_total_kernel(uint32_t *__restrict__ total) {
-  if (threadIdx.x == 0) total[1] = total[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1008,9 +1054,12 @@ struct TilesEngine {
   uint32_t *hist = nullptr, *seg = nullptr, *cursor = nullptr;
   Desc *desc = nullptr;        // work descriptors of the round
   uint32_t *ndesc = nullptr;
-  uint32_t *total = nullptr;   // [2]: packets of the round being planned
+  // [3]: packets of the round being planned, packets of the round just
+  // processed, number of plans that found packets
+  uint32_t *total = nullptr;
   unsigned *ticket = nullptr;
-  uint32_t *h_total = nullptr;  // pinned
+  uint32_t *h_total = nullptr;  // pinned, [3]
+  uint32_t *src_off = nullptr;  // each source's offset in its round-0 tile segment
   int tile_key[3] = {0, 0, 0};
   std::vector<hipEvent_t> ev;   // start/stop of every process launch
   // problem set
@@ -1047,6 +1096,7 @@ static void free_engine(TilesEngine *e) {
   hipFree(e->ndesc);
   hipFree(e->total);
   hipFree(e->ticket);
+  hipFree(e->src_off);
   if (e->h_total) hipHostFree(e->h_total);
   hipFree(e->d_probs);
   hipFree(e->d_tile_prob);
@@ -1156,9 +1206,10 @@ static int ensure_engine(cmi_ctx *c, const std::vector<ProbGeo> &probs,
   if (err == hipSuccess) err = hipMalloc(&e->cursor, ntiles * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->desc, e->desc_cap * sizeof(Desc));
   if (err == hipSuccess) err = hipMalloc(&e->ndesc, sizeof(uint32_t));
-  if (err == hipSuccess) err = hipMalloc(&e->total, 2 * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->total, 3 * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->ticket, sizeof(unsigned));
-  if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 2 * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 3 * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->src_off, kMaxSetSources * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->d_probs, np * sizeof(ProbGeo));
   if (err == hipSuccess) err = hipMalloc(&e->d_tile_prob, ntiles * sizeof(uint16_t));
   if (err == hipSuccess) err = hipMalloc(&e->d_src, sizeof(SetSources));
@@ -1269,62 +1320,61 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
 #undef CMI_PROC
 }
 
+// nonempty: count the plans that find packets (total[2]) — the rounds after
+// round 0 that had work.
 static void launch_plan(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
-                        bool track) {
+                        bool track, bool nonempty = false) {
   TilesEngine *e = c->tiles;
   hipLaunchKernelGGL(tiles_plan_kernel, dim3(1), dim3(kPlanThreads), 0, c->stream,
                      G.ntiles, e->hist, e->seg, e->cursor, e->desc, e->ndesc,
                      e->total, e->ticket, R.chunk, e->d_tile_prob, e->nprob,
                      track ? e->d_prob_round : nullptr, e->d_prob_time,
-                     e->d_round_ctr);
+                     e->d_round_ctr, nonempty ? e->total + 2 : nullptr);
 }
 
-// Rounds of process / plan / place until no packet is left (host-checked
-// after every round).  Round 0's input (records, destinations, histogram)
-// is already in P[0], dest[0], hist; n0 packets.
-static int run_rounds(cmi_ctx *c, const TileGeo &G, size_t n0, bool with_pid,
+// The host checks the packets left every kRoundsPerCheck rounds: a check
+// costs a host round trip (about 30 us), a round launched after the last
+// packet has gone about 20 us (its kernels find no work).
+constexpr int kRoundsPerCheck = 8;
+
+// Rounds of process / plan / place until no packet is left.  Round 0 is
+// planned and its index list built (records in P[0]); n0 packets.
+static int run_rounds(cmi_ctx *c, const TileGeo &G, const RoundCfg &R, bool with_pid,
                       PacketOut po, uint32_t *rounds_out) {
   TilesEngine *e = c->tiles;
-  const RoundCfg R = round_cfg(c, G, n0);
-  if (R.lds > kLdsBytes)
-    return fail(c, CMI_ERR_ARG, "tiles engine: the tile's opacity and accumulator exceed the LDS");
-  launch_plan(c, G, R, false);
-  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
-                     0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
   // straight rays visit a bounded number of tiles; reemitted packets turn,
   // so their rounds are bounded only by the cap
   const int max_rounds = G.reemit_p > 0.f ? kReemitRoundCap : max_rounds_of(e->probs) + 2;
-  const int timed = std::min<int>(max_rounds, (int)(e->ev.size() / 2));
+  const int timed = c->kernel_timing ? std::min<int>(max_rounds, (int)(e->ev.size() / 2)) : 0;
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, c->stream, e->total + 2,
+                     (size_t)1);
   int r = 0;
-  for (; r < max_rounds; ++r) {
-    // round r reads P[0] (through idx) and writes its survivors to P[1];
-    // the buffers swap roles before the next round
-    if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r], c->stream));
-    launch_process(c, G, R, with_pid, po, nullptr);
-    if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r + 1], c->stream));
-    CMI_TRY_HIP(c, hipGetLastError());
-    // total[0] = packets processed this round = slots written to P[1];
-    // keep it in total[1] for the placement, then plan the next round
-    hipLaunchKernelGGL(copy_total_kernel, dim3(1), dim3(64), 0, c->stream, e->total);
-    launch_plan(c, G, R, false);
-    CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, sizeof(uint32_t),
+  bool drained = false;
+  while (r < max_rounds && !drained) {
+    const int r_end = std::min(max_rounds, r + kRoundsPerCheck);
+    for (; r < r_end; ++r) {
+      // round r reads P[0] (through idx) and writes its survivors to P[1];
+      // the plan keeps the round's count in total[1] for the placement
+      if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r], c->stream));
+      launch_process(c, G, R, with_pid, po, nullptr);
+      if (r < timed) CMI_TRY_HIP(c, hipEventRecord(e->ev[2 * r + 1], c->stream));
+      CMI_TRY_HIP(c, hipGetLastError());
+      launch_plan(c, G, R, false, true);
+      hipLaunchKernelGGL(tiles_place_kernel, dim3(R.place_blocks), dim3(256), 0, c->stream,
+                         e->total + 1, e->dest[1], e->rel, e->seg, e->idx);
+      CMI_TRY_HIP(c, hipGetLastError());
+      std::swap(e->P[0], e->P[1]);
+      std::swap(e->dest[0], e->dest[1]);
+      if (with_pid) std::swap(e->pid[0], e->pid[1]);
+    }
+    CMI_TRY_HIP(c, hipMemcpyAsync(e->h_total, e->total, 3 * sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, c->stream));
     CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
-    if (e->h_total[0] == 0) {
-      ++r;
-      break;
-    }
-    hipLaunchKernelGGL(tiles_place_kernel, dim3(R.place_blocks), dim3(256), 0, c->stream,
-                       e->total + 1, e->dest[1], e->rel, e->seg, e->idx);
-    CMI_TRY_HIP(c, hipGetLastError());
-    std::swap(e->P[0], e->P[1]);
-    std::swap(e->dest[0], e->dest[1]);
-    if (with_pid) std::swap(e->pid[0], e->pid[1]);
+    drained = e->h_total[0] == 0;
   }
-  if (rounds_out) *rounds_out = (uint32_t)r;
-  if (r >= max_rounds) return fail(c, CMI_ERR_DIRECTION, "tiles engine: round limit reached");
+  if (rounds_out) *rounds_out = 1u + e->h_total[2];
+  if (!drained) return fail(c, CMI_ERR_DIRECTION, "tiles engine: round limit reached");
   // device time of every process launch (the dominant kernel)
-  CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
   const int nt = std::min(r, timed);
   for (int k = 0; k < nt; ++k) {
     float ms = 0.f;
@@ -1383,17 +1433,23 @@ int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   fill_set_sources_single(c, e, seed, iteration);
   CMI_TRY_HIP(c, hipMemcpyAsync(e->d_src, e->h_src, sizeof(SetSources),
                                 hipMemcpyHostToDevice, c->stream));
+  const RoundCfg R = round_cfg(c, G, (size_t)npackets);
+  if (R.lds > kLdsBytes)
+    return fail(c, CMI_ERR_ARG, "tiles engine: the tile's opacity and accumulator exceed the LDS");
   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt);
   CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
+  // round 0: per-tile counts from the source split, planned, then the
+  // emission fills the records and the index list
+  hipLaunchKernelGGL(tiles_source_hist_kernel, dim3(1), dim3(256), 0, c->stream, G,
+                     e->d_src, e->hist, e->src_off);
+  launch_plan(c, G, R, false);
   const unsigned eb = (unsigned)((npackets + 255) / 256);
   hipLaunchKernelGGL(tiles_emit_kernel, dim3(eb), dim3(256), 0, c->stream, G, e->d_src,
-                     npackets, e->P[0], e->dest[0]);
-  hipLaunchKernelGGL(tiles_source_hist_kernel, dim3(1), dim3(256), 0, c->stream, G,
-                     e->d_src, e->hist);
+                     npackets, e->P[0], e->seg, e->src_off, e->idx);
   CMI_TRY_HIP(c, hipGetLastError());
   PacketOut po{};
   uint32_t rounds = 0;
-  rc = run_rounds(c, G, (size_t)npackets, false, po, &rounds);
+  rc = run_rounds(c, G, R, false, po, &rounds);
   if (rc != CMI_OK) return rc;
   CMI_TRY_HIP(c, hipEventRecord(c->ev1, c->stream));
   CMI_TRY_HIP(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(Counters),
@@ -1454,9 +1510,18 @@ int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub
   hipLaunchKernelGGL(tiles_load_list_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
                      0, c->stream, G, c->g, dp, ds, n, e->P[0], e->dest[0], e->pid[0],
                      e->hist);
+  const RoundCfg R = round_cfg(c, G, n);
+  if (R.lds > kLdsBytes) {
+    cleanup();
+    return fail(c, CMI_ERR_ARG, "tiles engine: the tile's opacity and accumulator exceed the LDS");
+  }
+  // round 0: planned from the loaded histogram, index list by destination
+  launch_plan(c, G, R, false);
+  hipLaunchKernelGGL(tiles_scatter_kernel, dim3(R.scatter_blocks), dim3(kScatterThreads),
+                     0, c->stream, e->total, e->dest[0], e->seg, e->cursor, e->idx);
   PacketOut po{dst, dc, dh, fa, fb, dtile};
   uint32_t rounds = 0;
-  rc = run_rounds(c, G, n, true, po, &rounds);
+  rc = run_rounds(c, G, R, true, po, &rounds);
   if (rc != CMI_OK) {
     cleanup();
     return rc;

@@ -65,6 +65,8 @@ class Problem:
         self.nsub = [nsub] * 3
         self.ctx = Context(box, ncell, self.nsub, PREC_FP32)
         self.ctx.set_engine(ENGINE_TILES)
+        # convergence runs do not read the per-launch kernel timing
+        self.ctx.set_kernel_timing(False)
         self.n_ref = W.global_to_subgrid_major(n_lattice, self.nsub)
         self.xn0 = np.full(self.n_ref.size, W.XN0)
         self.ctx.set_sources(pos, L)

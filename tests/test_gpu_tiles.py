@@ -86,6 +86,58 @@ def test_tiles_transport_matches_walk(oracle):
     assert np.median(rel) < 1e-5
 
 
+def test_tiles_round_count_and_timing_switch():
+    """The host checks the packets left only every few rounds; the reported
+    round count is still the number of rounds that had packets (= the most
+    tile visits of any packet), and switching the per-launch timing off
+    changes nothing but the timing record."""
+    ncell, nsub = [64, 64, 64], [4, 4, 4]
+    box = W.box_10pc()
+    n, xn = _field(6, ncell, nsub, -6, -3)
+    pos, L = W.random_sources(4, box, seed=9)
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as c64:
+        c64.set_sources(pos, L)
+        pk, ss = c64.emit(20000, 2, 0)
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+        ctx.set_engine(cmi.ENGINE_TILES, tile=(8, 8, 8))
+        ctx.set_fields(n, xn)
+        _, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+        assert stats["errors"] == 0
+        assert stats["rounds"] == int(hp.max())
+        ctx.set_sources(pos, L)
+        ctx.reset_intensity()
+        ctx.kernel_timing(reset=True)
+        s_on = ctx.transport(300000, 4, 2)
+        J_on, _ = ctx.get_fields()
+        avg, calls = ctx.kernel_timing(reset=True)
+        assert calls >= s_on["rounds"] and avg > 0
+        ctx.set_kernel_timing(False)
+        ctx.reset_intensity()
+        s_off = ctx.transport(300000, 4, 2)
+        J_off, _ = ctx.get_fields()
+        assert ctx.kernel_timing()[1] == 0
+        for k in ("crossings", "absorbed", "escaped", "rounds", "hops"):
+            assert s_on[k] == s_off[k], k
+        assert np.allclose(J_on, J_off, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("prec", [cmi.PREC_FP32, cmi.PREC_FP64])
+def test_reset_intensity_clears_nonfinite(prec):
+    """A context's J comes from freshly allocated device memory, which may
+    hold NaN or Inf bit patterns: the reset stores zeros rather than scaling
+    by zero (which would keep them and make every later update of those cells
+    fully neutral)."""
+    ncell = [32, 32, 32]
+    with cmi.Context(W.box_10pc(), ncell, [2, 2, 2], prec) as ctx:
+        J = np.full(32 ** 3, np.nan)
+        J[::3] = np.inf
+        J[1::3] = -np.inf
+        ctx.set_intensity(J)
+        ctx.reset_intensity()
+        Jg, _ = ctx.get_fields()
+        assert np.all(Jg == 0.0)
+
+
 def test_tiles_stromgren_radius():
     box = W.box_10pc()
     ncell, nsub = [64, 64, 64], [4, 4, 4]
@@ -96,7 +148,10 @@ def test_tiles_stromgren_radius():
         for it in range(16):
             s = ctx.transport(N, 1, it)
             assert s["errors"] == 0
+            assert s["absorbed"] + s["escaped"] == N
             l1 = ctx.update_ionization(W.Q_STROMGREN, N, W.ALPHA_B, form=1)
+            print("it %d: absorbed %d crossings %d rounds %d l1 %.3e"
+                  % (it, s["absorbed"], s["crossings"], s["rounds"], l1))
         _, x = ctx.get_fields()
     R = W.ionized_radius(x, box, ncell)
     Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)

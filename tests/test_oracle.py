@@ -458,6 +458,24 @@ def test_source_counts(oracle):
     assert ((out - base) >= 0).all() and ((out - base) <= 1).all()
 
 
+def test_source_counts_sum_exactly(oracle):
+    # the emission files packets by this split: it must sum to N for any
+    # luminosities, including ones whose rounded shares sit on integers
+    rng = np.random.default_rng(5)
+    cases = [(np.full(3, 1.0 / 3.0), 3_000_000), (np.full(7, 0.1), 70),
+             (np.array([1e-300, 1.0]), 10)]
+    for _ in range(200):
+        ns = int(rng.integers(1, 40))
+        cases.append((10.0 ** rng.uniform(40, 50, ns), int(rng.integers(1, 10**9))))
+    for L, n in cases:
+        out = np.zeros(len(L), np.uint64)
+        oracle.orc_source_counts(O.ptr(np.ascontiguousarray(L), O._dp), len(L), n, 3, 1,
+                                 O.ptr(out, O._u64p))
+        assert int(out.sum()) == n
+        share = L / L.sum() * n
+        assert (np.abs(out - share) <= 1.0 + 1e-6 * share).all()
+
+
 # ---- the reference's other two walk variants -------------------------------
 
 def _walk_oracle(oracle, box, ncell, n, xn, photons, in_dir, kind):
