@@ -347,11 +347,12 @@ __device__ __forceinline__ uint64_t block_exclusive_scan(uint64_t v,
   return r;
 }
 
-// A tile with more than kPlanSerialChunks chunks has its descriptors written
-// by the whole workgroup (a source's tile holds millions of packets in the
-// first rounds); up to kPlanCrowded such tiles per plan, the rest serially.
-constexpr uint32_t kPlanSerialChunks = 4;
-constexpr int kPlanCrowded = 512;
+// A tile with more than kPlanSerialChunks chunks is crowded: the descriptors
+// of up to kPlanCrowded crowded tiles are written by the whole workgroup from
+// one flattened range (a tile found by binary search over the crowded tiles'
+// chunk prefix in LDS); the other tiles' descriptors by their own thread.
+constexpr uint32_t kPlanSerialChunks = 2;
+constexpr int kPlanCrowded = 1024;
 
 // total[0]: packets of the round being planned (in: those of the round just
 // processed, kept in total[1] for the placement); nonempty (optional) counts
@@ -370,6 +371,7 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
   __shared__ uint32_t s_round;
   __shared__ uint32_t s_ncrowded;
   __shared__ uint4 s_crowded[kPlanCrowded];  // tile, first descriptor, count, segment start
+  __shared__ uint32_t s_cpre[kPlanCrowded];  // exclusive prefix of their chunks
   const bool track = prob_round != nullptr;
   if (threadIdx.x == 0) {
     total[1] = total[0];
@@ -416,10 +418,23 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
   }
   __syncthreads();
   const uint32_t ncrowded = min(s_ncrowded, (uint32_t)kPlanCrowded);
-  for (uint32_t b = 0; b < ncrowded; ++b) {
-    const uint4 B = s_crowded[b];
-    const uint32_t nch = (B.z + chunk - 1) / chunk;
-    for (uint32_t j = threadIdx.x; j < nch; j += kPlanThreads) {
+  if (ncrowded) {
+    static_assert(kPlanCrowded <= kPlanThreads, "one crowded tile per thread in the scan");
+    const uint64_t mine = threadIdx.x < ncrowded
+                              ? (s_crowded[threadIdx.x].z + chunk - 1) / chunk : 0;
+    uint64_t ncd;
+    const uint64_t pre = block_exclusive_scan(mine, sh, &ncd);
+    if (threadIdx.x < ncrowded) s_cpre[threadIdx.x] = (uint32_t)pre;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < (uint32_t)ncd; k += kPlanThreads) {
+      // the last crowded tile whose first flattened descriptor is <= k
+      uint32_t lo = 0, hi = ncrowded - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (s_cpre[mid] <= k) lo = mid; else hi = mid - 1;
+      }
+      const uint4 B = s_crowded[lo];
+      const uint32_t j = k - s_cpre[lo];
       Desc d;
       d.tile = (int)B.x;
       d.start = B.w + j * chunk;
