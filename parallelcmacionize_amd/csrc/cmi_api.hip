@@ -47,7 +47,7 @@ inline size_t dev_index(const cmi_ctx *c, size_t sub, int ix, int iy, int iz) {
   const int lx = ix - (tx << g.tshift[0]), ly = iy - (ty << g.tshift[1]),
             lz = iz - (tz << g.tshift[2]);
   const size_t t = ((size_t)tx * g.ntile[1] + ty) * g.ntile[2] + tz;
-  const size_t l = (((size_t)lx << g.tshift[1]) + ly << g.tshift[2]) + lz;
+  const size_t l = ((((size_t)lx << g.tshift[1]) + ly) << g.tshift[2]) + lz;
   return sub * (size_t)g.cps + t * (size_t)g.tile_cells + l;
 }
 

@@ -878,7 +878,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         continue;
       }
       tau -= dtau;
-      t = fmaxf(t, tn);
+      t = tn > t ? tn : t;  // fmaxf without the NaN canonicalisation
       const bool a0 = tm0 == tn;
       const bool a1 = !a0 && (tm1 == tn);
       const bool a2 = !a0 && !a1;
@@ -892,12 +892,13 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const bool out0 = (unsigned)l0 >= (unsigned)T0, out1 = (unsigned)l1 >= (unsigned)T1,
                  out2 = (unsigned)l2 >= (unsigned)T2;
       const bool out = out0 | out1 | out2;
-      // walk bound: crossings of this visit (of this flight after a
-      // reemission)
+      // walk bound, for reemitted flights only: every other step moves one
+      // cell along one axis, always in the direction's sign along that axis,
+      // so a flight leaves the tile within T0 + T1 + T2 steps
       if (kReemit) ++steps;
       const int nsteps = kReemit ? steps : (int)vis_cross;
-      if (!out) kap_c = kapL[off];  // the next cell, read ahead
-      if (out | (nsteps > max_steps)) {
+      kap_c = kapL[out ? 0 : off];  // the next cell, read ahead (unused if out)
+      if (out | (kReemit && nsteps > max_steps)) {
         if (out) {
           exit_axis = out0 ? 0 : (out1 ? 1 : 2);
           const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
