@@ -21,9 +21,12 @@
 //             segments, and the next round's work descriptors (tile, start,
 //             count) in chunks;
 //   place     segment start + relative position of every survivor -> the
-//             index list the next round gathers packets through (the
-//             emission round, which has no visits, is scattered with per-block
-//             LDS counts and per-tile cursors instead).
+//             index list the next round gathers packets through.
+//
+// Round 0 has no visits before it: for emitted packets its per-tile counts
+// follow from the source split, so it is planned before the emission, which
+// writes every packet's index-list entry itself; an explicit packet list is
+// scattered by destination (per-block LDS counts, per-tile cursors).
 //
 // Refills are batched per wave: finished lanes wait until a threshold of
 // lanes is free (or the wave runs dry) and then write out and reload
