@@ -81,7 +81,7 @@ def test_forward_matches_torch(net):
     assert np.abs(z - zt).max() < 2e-2 * scale, (np.abs(z - zt).max(), scale)
 
 
-@pytest.mark.parametrize("dil", [(1, 1, 1, 1), (1, 2, 4, 8)])
+@pytest.mark.parametrize("dil", [(1, 1, 1, 1), (1, 2, 4, 8), (2, 4, 8, 16)])
 def test_gradients_match_torch(net, dil):
     rng = np.random.default_rng(1)
     B = 2
