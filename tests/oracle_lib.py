@@ -240,3 +240,61 @@ def ref_walk_list(ref, box, ncell, n, xn, photons, input_direction, kind):
                       p.ctypes.data, m, input_direction, kind, ptr(ex, _i32p),
                       ptr(cr, _u32p))
     return p, ex, cr
+
+
+def degenerate_packets(box, ncell, offset, rng, m_random=4):
+    """Packets whose walks tie: the 26 directions along the axes, the face
+    diagonals and the body diagonals, from cell centres, grid points (cell
+    corners) and face centres, plus a few random interior points.  With a box
+    of power-of-two size and ncell a power of two every cell face lies on an
+    exactly representable coordinate, so the ties are exact.  offset (the
+    entry direction's offset, -1/0/+1 per axis): the packets start on that
+    side of the box and point inward along it."""
+    box = np.asarray(box, dtype=np.float64)
+    ncell = np.asarray(ncell)
+    dx = box[3:] / ncell
+    s2, s3 = 1.0 / np.sqrt(2.0), 1.0 / np.sqrt(3.0)
+    dirs = []
+    for a in range(3):
+        for sg in (1.0, -1.0):
+            d = np.zeros(3)
+            d[a] = sg
+            dirs.append(d)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        for sa in (1.0, -1.0):
+            for sb in (1.0, -1.0):
+                d = np.zeros(3)
+                d[a], d[b] = sa * s2, sb * s2
+                dirs.append(d)
+    for sx in (1.0, -1.0):
+        for sy in (1.0, -1.0):
+            for sz in (1.0, -1.0):
+                dirs.append(np.array([sx, sy, sz]) * s3)
+    starts = []
+    for _ in range(3):
+        c = rng.integers(0, ncell)
+        starts.append(box[:3] + (c + 0.5) * dx)          # cell centre
+        g = rng.integers(1, ncell)
+        starts.append(box[:3] + g * dx)                  # grid point
+        f = (c + 0.5) * dx
+        a = rng.integers(0, 3)
+        f[a] = g[a] * dx[a]
+        starts.append(box[:3] + f)                       # on a cell face
+    for _ in range(m_random):
+        starts.append(box[:3] + rng.uniform(0, 1, 3) * box[3:])
+    p = np.zeros(len(dirs) * len(starts), PHOTON_DTYPE)
+    k = 0
+    for s in starts:
+        for d in dirs:
+            pos, dd = s.copy(), d.copy()
+            for a in range(3):
+                if offset[a]:
+                    pos[a] = box[a] + (box[3 + a] if offset[a] > 0 else 0.0)
+                    dd[a] = -offset[a] * abs(dd[a])
+            p[k]["pos"] = pos
+            p[k]["dir"] = dd
+            k += 1
+    p["tau"] = rng.exponential(3.0, len(p))
+    p["sigma"] = 6.3e-22
+    p["weight"] = rng.uniform(0.5, 2.0, len(p))
+    return p

@@ -77,6 +77,27 @@ def test_interact_batch_matches_oracle_anisotropic(oracle, in_dir):
     _close_J(J, Jo)
 
 
+@pytest.mark.parametrize("thick", [False, True])
+@pytest.mark.parametrize("k", [0, 1])
+def test_interact_batch_degenerate_matches_oracle(oracle, k, thick):
+    """Tied walks (axes, face and body diagonals from cell centres, grid
+    points and faces; exact cell faces in a power-of-two box) through the
+    FP64 walk: exits, crossings, positions and optical depths bit-exact."""
+    from test_oracle import DEGENERATE_IN_DIRS, _interact_oracle, degenerate_case
+    for in_dir in DEGENERATE_IN_DIRS:
+        box, ncell, n, xn, p = degenerate_case(oracle, k, in_dir, thick)
+        print("degenerate case", k, thick, in_dir, flush=True)
+        po, exo, cro, Jo = _interact_oracle(oracle, box, ncell, n, xn, p, in_dir)
+        with cmi.Context(box, ncell, [1, 1, 1], cmi.PREC_FP64) as ctx:
+            ctx.set_fields(n, xn)
+            pg, exg, crg = ctx.interact_batch(0, in_dir, p)
+            J, _ = ctx.get_fields()
+        assert (exg == exo).all(), in_dir
+        assert (crg == cro).all(), in_dir
+        assert _bitwise_equal(pg, po), in_dir
+        _close_J(J, Jo)
+
+
 @pytest.mark.parametrize("tile", [None, (8, 8, 8), (4, 8, 16)])
 def test_transport_packets_matches_golden(golden, tile):
     g = golden("transport_32.npz")

@@ -236,6 +236,46 @@ def test_interact_against_live_reference(oracle, ref):
         assert (J == Jr).all()
 
 
+DEGENERATE_BOXES = [
+    # power-of-two sizes and cell counts: every cell face is exactly
+    # representable, so diagonal walks tie exactly at cell edges and corners
+    (np.array([-2.0 ** 54, 0.0, 2.0 ** 52, 2.0 ** 55, 2.0 ** 55, 2.0 ** 55]), [16, 16, 16]),
+    (np.array([2.0 ** 53, -2.0 ** 55, 0.0, 2.0 ** 54, 2.0 ** 55, 2.0 ** 53]), [8, 32, 4]),
+]
+DEGENERATE_IN_DIRS = (0, 1, 8, 11, 14, 19, 21, 26)
+
+
+def degenerate_case(oracle, k, in_dir, thick=False):
+    box, ncell = DEGENERATE_BOXES[k]
+    rng = np.random.default_rng(1000 * k + 100 * thick + in_dir)
+    nc = int(np.prod(ncell))
+    n = 1e8 * np.exp(rng.normal(0, 1.0, nc))
+    # thin: the diagonal walks reach the box's edges and corners; thick: most
+    # packets are absorbed on the way (partial last steps on tied walks)
+    xn = 10 ** (rng.uniform(-7, -1, nc) if thick else rng.uniform(-9, -5, nc))
+    off = np.zeros(3, np.int32)
+    oracle.orc_direction_offset(in_dir, O.ptr(off, O._ip))
+    p = O.degenerate_packets(box, ncell, off, rng)
+    return box, np.array(ncell, np.int32), n, xn, p
+
+
+@pytest.mark.parametrize("thick", [False, True])
+@pytest.mark.parametrize("k", [0, 1])
+def test_degenerate_packets_against_live_reference(oracle, ref, k, thick):
+    """Walks along axes, face and body diagonals from cell centres, grid
+    points and faces, entering through faces, edges and corners: the
+    reference's tie-breaks and face snapping, bit for bit."""
+    for in_dir in DEGENERATE_IN_DIRS:
+        box, ncell, n, xn, p = degenerate_case(oracle, k, in_dir, thick)
+        q = p.copy()
+        po, ex, cr, J = _interact_oracle(oracle, box, ncell, n, xn, p, in_dir)
+        Jr, exr, crr = O.ref_interact_list(ref, box, ncell, n, xn, q, in_dir)
+        assert (ex == exr).all(), in_dir
+        assert (cr == crr).all(), in_dir
+        assert (po.view(np.uint8) == q.view(np.uint8)).all(), in_dir
+        assert (J == Jr).all(), in_dir
+
+
 def test_neutral_fraction_against_live_reference(oracle, ref):
     rng = np.random.default_rng(12)
     box = np.array([0., 0., 0., 2 * PC, 1 * PC, 3 * PC])
