@@ -165,6 +165,64 @@ def test_transport_emitted_matches_oracle(oracle):
     _close_J(J, og.J)
 
 
+@pytest.mark.parametrize("engine", ["walk", "tiles"])
+def test_ragged_grid_transport_matches_oracle(oracle, engine):
+    """A grid of 48 x 32 x 16 cells in 3 x 2 x 1 subgrids over a 3 x 2 x 1 pc
+    box: the walk (FP64) bit-exact, the tiles engine (FP32, 8 x 16 x 4 tiles)
+    to the FP32 bars."""
+    ncell, nsub = [48, 32, 16], [3, 2, 1]
+    rng = np.random.default_rng(21)
+    box = np.array([-1.5 * PC, -1.0 * PC, -0.5 * PC, 3.0 * PC, 2.0 * PC, 1.0 * PC])
+    og = O.OracleGrid(oracle, box, ncell, nsub)
+    og.n[:] = 1e8 * np.exp(rng.normal(0, 1, og.n.size))
+    og.xn[:] = 10 ** rng.uniform(-6, -2, og.n.size)
+    og.J[:] = 0
+    pos, L = W.random_sources(4, box, seed=8)
+    prec = cmi.PREC_FP64 if engine == "walk" else cmi.PREC_FP32
+    N = 40000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as c64:
+        c64.set_sources(pos, L)
+        pk, ss = c64.emit(N, 6, 0)
+    with cmi.Context(box, ncell, nsub, prec) as ctx:
+        if engine == "tiles":
+            ctx.set_engine(cmi.ENGINE_TILES, tile=(8, 16, 4))
+        ctx.set_fields(og.n, og.xn)
+        p, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+        J, _ = ctx.get_fields()
+    q = pk.copy()
+    sto, cro, hpo = og.transport(q, ss, nthreads=4)
+    assert stats["errors"] == 0
+    if engine == "walk":
+        assert (st == sto).all() and (cr == cro).all() and (hp == hpo).all()
+        assert _bitwise_equal(p, q)
+        _close_J(J, og.J)
+    else:
+        assert (st == sto).mean() > 0.999
+        same = st == sto
+        assert np.abs(cr[same].astype(int) - cro[same].astype(int)).max() <= 2
+        assert abs(J.sum() / og.J.sum() - 1) < 1e-5
+
+
+@pytest.mark.parametrize("engine", ["walk", "tiles"])
+def test_empty_inputs(engine):
+    """Empty packet lists are no-ops, as the reference's empty buffers are."""
+    prec = cmi.PREC_FP64 if engine == "walk" else cmi.PREC_FP32
+    with cmi.Context(W.box_10pc(), [16, 16, 16], [1, 1, 1], prec) as ctx:
+        if engine == "tiles":
+            ctx.set_engine(cmi.ENGINE_TILES)
+        empty = np.zeros(0, O.PHOTON_DTYPE)
+        p, st, cr, hp, stats = ctx.transport_packets(empty, np.zeros(0, np.uint32))
+        assert len(p) == 0 and stats["crossings"] == 0
+        if engine == "walk":
+            p, ex, cr = ctx.interact_batch(0, 0, empty)
+            assert len(p) == 0 and len(ex) == 0
+        ctx.set_sources([[0.0, 0.0, 0.0]], [1e49])
+        pk, ss = ctx.emit(100, 1, 0, count=0)
+        assert len(pk) == 0 and len(ss) == 0
+        J, _ = ctx.get_fields()
+        assert np.all(J == 0)
+
+
 @pytest.mark.parametrize("form", [0, 1])
 def test_ionization_matches_oracle(oracle, golden, form):
     g = golden("neutral_fraction_16.npz")
