@@ -35,6 +35,10 @@ FEATURE_MEAN = (8.0, -7.0)
 FEATURE_SCALE = (2.0, 0.5)
 # dilations of the four 3x3x3 layers: a receptive field of 31^3 cells
 DILATION = (1, 2, 4, 8)
+# held-out problems (field, source and luminosity seeds) of the 256^3 clumpy
+# kind: neither the training problems (seeds 1000+, 2000+, 3000+) nor the
+# 256^3 clumpy workload (2, 3, 4)
+HELDOUT = ((999, 998, 997), (995, 994, 993), (990, 989, 988), (985, 984, 983))
 
 
 def logit(x):
