@@ -339,13 +339,16 @@ int cmi_net_loss_and_grad(cmi_net *net, const float *features,
                           const int dim[3], float *loss, float *grad);
 int cmi_net_predict(cmi_net *net, const float *features, const int dim[3],
                     float *logits);
-/* Warm start of a context, all on the device.  The context's J must hold the
- * deposits of a first transport of `npackets` packets at total luminosity
- * `luminosity`; per cell jH = L / N / V_cell * J (the jfac of
- * cmi_update_ionization).  Features: log10 n_H and log10(jH + 1e-12 s^-1).
- * The network's logits set x_n = 1 / (1 + exp(-logit)), clamped to
- * [1e-7, 1]; J is zeroed and the FP32 opacity refreshed.  ncell[2] of the
- * context must be a multiple of 16. */
+/* Warm start of a context, all on the device.  Input mode 0: the context's J
+ * must hold the deposits of a first transport of `npackets` packets at total
+ * luminosity `luminosity` (per cell jH = L / N / V_cell * J, the jfac of
+ * cmi_update_ionization); features log10 n_H and log10(jH + 1e-12 s^-1); the
+ * network's output is the logit of x_n.  Input mode 1: the context's x_n is
+ * the state after a few cheap iterations; features log10 n_H and the logit
+ * of that x_n (clamped to [1e-7, 1 - 1/150]); the output is added to that
+ * logit.  Either way x_n = 1 / (1 + exp(-logit)), clamped to [1e-7, 1], the
+ * start rule applies, J is zeroed and the FP32 opacity refreshed.  ncell[2]
+ * of the context must be a multiple of 16. */
 int cmi_net_warm_start(cmi_net *net, cmi_ctx *ctx, double luminosity,
                        uint64_t npackets);
 /* The warm start's rule for cells the network calls neutral: a cell whose
@@ -355,8 +358,8 @@ int cmi_net_warm_start(cmi_net *net, cmi_ctx *ctx, double luminosity,
 int cmi_net_set_start_rule(cmi_net *net, float logit_cut, double fallback_xn);
 int cmi_net_get_start_rule(const cmi_net *net, float *logit_cut,
                            double *fallback_xn);
-/* The same features of a context's current J, as [D][H][W][2] FP32 on the
- * host (for building training sets from contexts). */
+/* The same features of a context (by the input mode), as [D][H][W][2] FP32
+ * on the host (for building training sets from contexts). */
 int cmi_net_context_features(cmi_net *net, cmi_ctx *ctx, double luminosity,
                              uint64_t npackets, float *features);
 

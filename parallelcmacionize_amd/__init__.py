@@ -588,14 +588,16 @@ class WarmStartNet:
         return out
 
     def warm_start(self, ctx, luminosity, npackets):
-        """Set ctx's x_n from the network, given ctx's J from a first
-        transport of npackets packets (cmi_net_warm_start)."""
+        """Set ctx's x_n from the network (cmi_net_warm_start): input mode 0
+        reads ctx's J from a first transport of npackets packets, mode 1
+        ctx's x_n after a few cheap iterations of npackets packets."""
         self._check(self.lib.cmi_net_warm_start(self.h, ctx.h, float(luminosity),
                                                 int(npackets)))
 
     def context_features(self, ctx, luminosity, npackets):
-        """(log10 n_H, log10(jH + 1e-12)) of ctx's current J as a
-        (nx, ny, nz, 2) float32 lattice (cmi_net_context_features)."""
+        """The network's input features of ctx as a (nx, ny, nz, 2) float32
+        lattice (cmi_net_context_features): (log10 n_H, log10(jH + 1e-12))
+        of ctx's J in input mode 0, (log10 n_H, logit x_n) in mode 1."""
         shape = tuple(int(v) for v in ctx.ncell) + (2,)
         out = np.zeros(shape, np.float32)
         self._check(self.lib.cmi_net_context_features(

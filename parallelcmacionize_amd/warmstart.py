@@ -3,15 +3,19 @@
 The plain method (testDensitySubGrid_omp.cpp:2274-2434) starts every cell at
 x_n = 1e-6 (DensitySubGrid.hpp:549) and alternates a photon transport with the
 ionization-equilibrium update until the neutral fraction stops changing.  The
-warm start replaces the starting x_n only: a cheap first transport (1e6
-packets through the x_n = 1e-6 medium) gives the unattenuated
-photoionization rate jH per cell, the network maps (log10 n_H, log10 jH) to
-the logit of the converged x_n, and the plain iteration then runs from there
-and still decides convergence (mean |x_n change| per cell < tol).
+warm start replaces the starting x_n only.  The shipped network (input mode
+1) reads the state after PASSES cheap iterations of PASS_PACKETS packets
+(log10 n_H and the logit of that x_n) and adds a correction to that logit;
+the plain iteration then runs from there and still decides convergence (mean
+|x_n change| per cell < tol).  Input mode 0, the first design, reads one
+cheap transport through the x_n = 1e-6 medium instead (log10 n_H and log10 of
+the unattenuated photoionization rate jH) and predicts the logit itself; it
+cannot see the shadows that decide where the fronts are.
 
 Training data are fields the method itself converges: clumpy log-normal media
-at the 256^3 clumpy workload's cell size, cut into 64^3 crops (features: the
-first-pass (log10 n_H, log10 jH); target: logit of the converged x_n).
+at the 256^3 clumpy workload's cell size, cut into 64^3 crops (target: the
+logit of the converged x_n, or in mode 1 its difference from the cheap
+iterate's).
 """
 import json
 import time
