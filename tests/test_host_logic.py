@@ -1,6 +1,8 @@
 """Host-side logic that needs no GPU: the autotuner's coordinate descent, the
 batched workload's specification, the warm-start helpers and the workload
 generators."""
+import os
+
 import numpy as np
 import pytest
 
@@ -95,3 +97,15 @@ def test_layout_roundtrip():
 def test_stromgren_radius_closed_form():
     R = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)
     assert 4 / 3 * np.pi * R ** 3 * W.N_H ** 2 * W.ALPHA_B == pytest.approx(W.Q_STROMGREN)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The measurement runs on one GPU; asking for more exits with a message
+    (before any device is touched)."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
