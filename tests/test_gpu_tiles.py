@@ -58,6 +58,37 @@ def test_tiles_list_matches_fp64_oracle(oracle, tile):
     assert np.median(d) < 1e-4 * (box[3] / 32)
 
 
+@pytest.mark.parametrize("thick", [False, True])
+def test_tiles_tied_walks(oracle, thick):
+    """The tied walks of the oracle's pinning (axes, face and body diagonals
+    through exact cell faces, edges and corners) through the FP32 tiles
+    engine: it steps tied axes one at a time, so its crossing counts may be
+    larger, but the packets' fates and the deposited path lengths agree."""
+    from test_oracle import degenerate_case
+    box, ncell, n, xn, pk = degenerate_case(oracle, 0, 0, thick)
+    pk["weight"] = 1.0          # the FP32 engine folds sigma into kappa, weight 1
+    ss = np.zeros(len(pk), np.uint32)
+    og = O.OracleGrid(oracle, box, ncell, [1, 1, 1])
+    og.n[:] = n
+    og.xn[:] = xn
+    og.J[:] = 0
+    q = pk.copy()
+    sto, cro, hpo = og.transport(q, ss, nthreads=4)
+    with cmi.Context(box, ncell, [1, 1, 1], cmi.PREC_FP32) as ctx:
+        ctx.set_engine(cmi.ENGINE_TILES, tile=(8, 8, 8))
+        ctx.set_fields(n, xn)
+        p, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+        J, _ = ctx.get_fields()
+    assert stats["errors"] == 0
+    assert (st == sto).mean() >= 0.999
+    same = st == sto
+    assert (cr[same].astype(int) >= cro[same].astype(int) - 2).all()
+    assert abs(J.sum() / og.J.sum() - 1) < 1e-5
+    big = og.J > 1e-3 * og.J.max()
+    rel = np.abs(J[big] - og.J[big]) / og.J[big]
+    assert np.median(rel) < 1e-5
+
+
 def test_tiles_transport_matches_walk(oracle):
     """Emission + transport through the tiles engine and through the walk
     engine (both FP32) deposit the same intensity field."""
