@@ -1356,8 +1356,9 @@ static void launch_plan(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
 // packet has gone about 20 us (its kernels find no work).
 constexpr int kRoundsPerCheck = 8;
 
-// Rounds of process / plan / place until no packet is left.  Round 0 is
-// planned and its index list built (records in P[0]); n0 packets.
+// Rounds of process / plan / place until no packet is left.  On entry round
+// 0 is planned (R sized for its packets) and its index list built, with the
+// records in P[0].
 static int run_rounds(cmi_ctx *c, const TileGeo &G, const RoundCfg &R, bool with_pid,
                       PacketOut po, uint32_t *rounds_out) {
   TilesEngine *e = c->tiles;
