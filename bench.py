@@ -271,9 +271,9 @@ def main():
         copy_calibration(int(args.copy_calibration))
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU "
-              "path (see DESIGN.md for how the transport would shard)",
-              file=sys.stderr)
+        print("bench.py runs on one GPU only: the transport has no multi-GPU "
+              "path (its packets would split over GPUs with one J all-reduce "
+              "per iteration)", file=sys.stderr)
         sys.exit(2)
 
     import parallelcmacionize_amd as cmi

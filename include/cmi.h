@@ -290,11 +290,11 @@ int cmi_batch_iteration(cmi_batch *b, const uint64_t *npackets, uint64_t seed,
 
 /* ---- learned warm start ----
  * A 4-layer 3-D convolutional network (3x3x3 kernels, 32 channels, ReLU;
- * no counterpart in the reference) from (log10 n_H, log10 of the first-pass
- * photoionization rate jH, s^-1) to the logit of the converged neutral
- * fraction, trained and run on BF16 MFMA with FP32 accumulation.  The plain
- * iteration stays the judge of convergence; the network only supplies the
- * starting x_n.  Feature arrays are [..][D][H][W][2] (FP32), target and logit
+ * no counterpart in the reference) from the features of the input mode (log10
+ * n_H and the first-pass photoionization rate or the cheap iterate's x_n, see
+ * cmi_net_set_input_mode) to the logit of the converged neutral fraction,
+ * trained and run on BF16 MFMA with FP32 accumulation.  The plain iteration
+ * still decides convergence; the network only supplies the starting x_n.  Feature arrays are [..][D][H][W][2] (FP32), target and logit
  * arrays [..][D][H][W]; the last extent W must be a multiple of 32.  Master
  * parameters are FP32, per layer l: W[27][NOUT][32] (tap = 9 kd + 3 kh + kw,
  * output channel, input channel; zero beyond the real 2 or 32 input and 32 or
