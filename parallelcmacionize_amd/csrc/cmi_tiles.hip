@@ -634,10 +634,18 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   // hundred thousand crossings per launch); widened for the final sums
   uint32_t my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0, my_err = 0;
 
+  // a workgroup's first visit is the descriptor of its own index; later ones
+  // come from the ticket, counted on from the grid size (no start-of-launch
+  // queue of every workgroup on the one counter)
+  bool first = true;
   for (;;) {
-    if (threadIdx.x == 0) s_desc = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t di = s_desc;
+    uint32_t di = blockIdx.x;
+    if (!first) {
+      if (threadIdx.x == 0) s_desc = gridDim.x + atomicAdd(ticket, 1u);
+      __syncthreads();
+      di = s_desc;
+    }
+    first = false;
     if (di >= ndesc) break;
     const Desc D = desc[di];
     const int tile = D.tile;
