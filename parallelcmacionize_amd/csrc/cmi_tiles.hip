@@ -19,7 +19,7 @@
 //             gets its position relative to its destination's segment;
 //   plan      one workgroup: exclusive scan of the histogram into tile
 //             segments, and the next round's work descriptors (tile, start,
-//             count) in chunks;
+//             count) in chunks, smaller ones for rounds with few packets;
 //   place     segment start + relative position of every survivor -> the
 //             index list the next round gathers packets through.
 //
