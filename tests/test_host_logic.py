@@ -109,3 +109,32 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_traffic_record_matches_its_profile():
+    """The HBM traffic the bench reports comes from a committed record of the
+    calibrated counter runs: its bytes are the counters times their copy
+    calibration factors, the counters are the ones in the rocprofv3 summary,
+    and the record applies to the bench's default step counts."""
+    import json
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    prof = os.path.join(root, "profiles")
+    with open(os.path.join(prof, "tiles_process_kernel_256clumpy_traffic.json")) as f:
+        tr = json.load(f)
+    want = (tr["fetch_kib_per_launch"] * 1024 * tr["fetch_factor"] +
+            tr["write_kib_per_launch"] * 1024 * tr["write_factor"])
+    assert tr["hbm_bytes_per_launch"] == pytest.approx(want, abs=1.0)
+    assert 0 < tr["hbm_bytes_per_launch"] < tr["algorithmic_bytes_per_launch"]
+    with open(os.path.join(prof, "tiles_process_kernel_256clumpy.md")) as f:
+        md = f.read()
+    row = "| %s | %d | %.1f | %.1f |" % (tr["kernel"], tr["launches"],
+                                         tr["fetch_kib_per_launch"],
+                                         tr["write_kib_per_launch"])
+    assert row in md
+    with open(os.path.join(root, "bench.py")) as f:
+        src = f.read()
+    steps = int(re.search(r'"--steps", type=int, default=(\d+)', src).group(1))
+    warmup = int(re.search(r'"--warmup", type=int, default=(\d+)', src).group(1))
+    assert (tr["steps"], tr["warmup"]) == (steps, warmup)
+    assert "--steps %d --warmup %d" % (steps, warmup) in tr["command"]
