@@ -86,7 +86,10 @@ typedef struct {
   int refill_batch;  /* tiles: finished lanes a wave gathers before it writes
                         them out and reloads together (1..64) */
   int chunk_min;     /* tiles: packets of one tile one workgroup visit
-                        takes (a tile's packets are split in such chunks) */
+                        takes (a tile's packets are split in such chunks;
+                        >= 256, default 8192); a round with fewer packets
+                        than a quarter of the resident workgroups' worth of
+                        such chunks uses smaller ones (down to 256) */
   int walk_blocks;   /* walk: persistent grid size */
   int lds_fp32;      /* tiles: 1 = FP32 LDS accumulators for J (flushed into
                         FP64), 0 = FP64 */
