@@ -176,6 +176,21 @@ int cmi_transport_packets(cmi_ctx *ctx, cmi_photon *p,
                           uint32_t *crossings, uint32_t *hops,
                           cmi_stats *stats);
 
+/* The same transport as cmi_transport_packets on the walk engine (whatever
+ * cmi_set_engine selected), for callers that cannot allocate device memory:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  No counterpart in the reference.  Host arrays as in
+ * cmi_transport_packets; they are staged through a fixed device array the
+ * library holds on each GPU until the process ends, so a call allocates
+ * nothing.  At most CMI_NOALLOC_MAX_PACKETS packets per call (CMI_ERR_ARG
+ * beyond: packets are independent, split longer lists).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race. */
+#define CMI_NOALLOC_MAX_PACKETS 65536
+int cmi_transport_packets_noalloc(cmi_ctx *ctx, cmi_photon *p,
+                                  const uint32_t *start_sub, size_t n,
+                                  int32_t *status, uint32_t *crossings,
+                                  uint32_t *hops, cmi_stats *stats);
+
 /* Point sources (Source.hpp:57): positions (3 per source, m) and ionizing
  * luminosities (s^-1).  Up to 64 sources. */
 int cmi_set_sources(cmi_ctx *ctx, const double *pos, const double *L,

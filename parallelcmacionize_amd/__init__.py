@@ -26,6 +26,7 @@ ENGINE_WALK = 0
 ENGINE_TILES = 1
 INSIDE = 0
 NEIGHBOUR_OUTSIDE = 0xFFFFFFFF
+NOALLOC_MAX_PACKETS = 65536  # CMI_NOALLOC_MAX_PACKETS of include/cmi.h
 
 PHOTON_DTYPE = np.dtype([("pos", "<f8", 3), ("dir", "<f8", 3), ("tau", "<f8"),
                          ("sigma", "<f8"), ("weight", "<f8")])
@@ -88,6 +89,9 @@ SIGNATURES = {
                                       _i32p, _u32p]),
     "cmi_transport_packets": (C.c_int, [_vp, _vp, _u32p, C.c_size_t, _i32p,
                                          _u32p, _u32p, C.POINTER(Stats)]),
+    "cmi_transport_packets_noalloc": (C.c_int, [_vp, _vp, _u32p, C.c_size_t,
+                                                 _i32p, _u32p, _u32p,
+                                                 C.POINTER(Stats)]),
     "cmi_set_sources": (C.c_int, [_vp, _dp, _dp, C.c_int]),
     "cmi_emit": (C.c_int, [_vp, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                            C.c_uint32, C.c_double, _vp, _u32p]),
@@ -281,7 +285,10 @@ class Context:
                        _p(ex, _i32p), _p(cr, _u32p)))
         return p, ex, cr
 
-    def transport_packets(self, photons, start_sub):
+    def transport_packets(self, photons, start_sub, noalloc=False):
+        """Whole-grid transport of an explicit packet list.  noalloc=True
+        calls cmi_transport_packets_noalloc (walk engine, staging through
+        the library's fixed device array, at most NOALLOC_MAX_PACKETS)."""
         p = np.ascontiguousarray(photons, dtype=PHOTON_DTYPE).copy()
         ss = np.ascontiguousarray(start_sub, dtype=np.uint32)
         n = len(p)
@@ -289,9 +296,10 @@ class Context:
         cr = np.zeros(n, np.uint32)
         hp = np.zeros(n, np.uint32)
         s = Stats()
-        self._check(self.lib.cmi_transport_packets(
-            self.h, p.ctypes.data, _p(ss, _u32p), n, _p(st, _i32p),
-            _p(cr, _u32p), _p(hp, _u32p), C.byref(s)))
+        fn = (self.lib.cmi_transport_packets_noalloc if noalloc
+              else self.lib.cmi_transport_packets)
+        self._check(fn(self.h, p.ctypes.data, _p(ss, _u32p), n, _p(st, _i32p),
+                       _p(cr, _u32p), _p(hp, _u32p), C.byref(s)))
         return p, st, cr, hp, _stats_dict(s)
 
     def set_sources(self, pos, L):

@@ -159,6 +159,30 @@ void fill_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets, double seconds,
 }  // namespace
 
 namespace cmi {
+// Checks and set-up shared by the explicit-list transports: arguments, start
+// subgrids in range, and in FP32 mode the packets' cross section folded into
+// kappa and weight 1.
+int prepare_list_transport(cmi_ctx *c, const cmi_photon *p,
+                           const uint32_t *start_sub, size_t n) {
+  if (!c || (n && (!p || !start_sub))) return CMI_ERR_ARG;
+  for (size_t i = 0; i < n; ++i)
+    if (start_sub[i] >= c->nsubgrid)
+      return fail(c, CMI_ERR_ARG, "start subgrid out of range");
+  if (n == 0) return CMI_OK;
+  (void)hipSetDevice(c->device);
+  if (c->precision == CMI_PREC_FP32) {
+    int rc = ensure_sigma(c, p[0].sigma);
+    if (rc == CMI_OK) rc = check_packets_fp32(c, p, n);
+    if (rc != CMI_OK) return rc;
+  }
+  return CMI_OK;
+}
+
+void fill_ctx_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets,
+                    double seconds) {
+  fill_stats(c, st, packets, seconds, 0);
+}
+
 // n_H of a context in the reference layout (for the result writers)
 int get_density_ref(cmi_ctx *c, double *nH) {
   (void)hipSetDevice(c->device);
@@ -300,6 +324,8 @@ int cmi_create(const double box[6], const int ncell[3], const int nsub[3],
   std::vector<double> n0(c->ncell_total, 1.e8), x0(c->ncell_total, 1.e-6);
   int rc = cmi_set_fields(c, n0.data(), x0.data());
   if (rc == CMI_OK) rc = cmi_reset_intensity(c);
+  // the allocation-free entry point's scratch address on this device
+  if (rc == CMI_OK) rc = noalloc_setup(c);
   if (rc != CMI_OK) {
     cmi_destroy(c);
     return rc;
@@ -495,19 +521,11 @@ int cmi_optical_depth_batch(cmi_ctx *c, uint32_t subgrid, int entry_dir, cmi_pho
 int cmi_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub,
                           size_t n, int32_t *status, uint32_t *crossings,
                           uint32_t *hops, cmi_stats *stats) {
-  if (!c || (n && (!p || !start_sub))) return CMI_ERR_ARG;
-  for (size_t i = 0; i < n; ++i)
-    if (start_sub[i] >= c->nsubgrid)
-      return fail(c, CMI_ERR_ARG, "start subgrid out of range");
+  const int prc = prepare_list_transport(c, p, start_sub, n);
+  if (prc != CMI_OK) return prc;
   if (n == 0) {
     if (stats) memset(stats, 0, sizeof(*stats));
     return CMI_OK;
-  }
-  (void)hipSetDevice(c->device);
-  if (c->precision == CMI_PREC_FP32) {
-    int rc = ensure_sigma(c, p[0].sigma);
-    if (rc == CMI_OK) rc = check_packets_fp32(c, p, n);
-    if (rc != CMI_OK) return rc;
   }
   if (c->engine == CMI_ENGINE_TILES)
     return tiles_transport_packets(c, p, start_sub, n, status, crossings, hops,

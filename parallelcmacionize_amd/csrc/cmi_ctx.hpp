@@ -102,6 +102,16 @@ inline void source_counts(const std::vector<double> &L, uint64_t npackets,
   for (uint64_t r = 0; r < rem; ++r) counts[order[r % (uint64_t)nsrc]]++;
 }
 
+// Shared by the explicit-list transports (cmi_api.hip).
+int prepare_list_transport(cmi_ctx *c, const cmi_photon *p,
+                           const uint32_t *start_sub, size_t n);
+void fill_ctx_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets,
+                    double seconds);
+
+// Device address of the allocation-free entry point's scratch array on the
+// context's device (cmi_noalloc.hip); resolved once per device.
+int noalloc_setup(cmi_ctx *c);
+
 // Tiles engine (cmi_tiles.hip).
 int tiles_transport_packets(cmi_ctx *c, cmi_photon *p,
                             const uint32_t *start_sub, size_t n,

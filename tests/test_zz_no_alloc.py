@@ -1,0 +1,80 @@
+"""cmi_transport_packets_noalloc (staging through the library's fixed device
+array, no allocation per call) against cmi_transport_packets on the same
+packets: per-packet results identical, deposits equal up to the order of the
+atomic sums, on two calls in a row with the second one smaller."""
+import numpy as np
+import pytest
+
+import parallelcmacionize_amd as cmi
+from parallelcmacionize_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _medium(ncell, seed):
+    rng = np.random.default_rng(seed)
+    n = 1e8 * np.exp(rng.normal(0.0, 1.0, ncell ** 3))
+    xn = 10 ** rng.uniform(-4, -1, ncell ** 3)
+    return n, xn
+
+
+def _run(ctx, pk, ss, noalloc):
+    ctx.reset_intensity()
+    p, st, cr, hp, stats = ctx.transport_packets(pk, ss, noalloc=noalloc)
+    J, _ = ctx.get_fields()
+    return p, st, cr, hp, stats, J
+
+
+@pytest.mark.parametrize("prec", [cmi.PREC_FP64, cmi.PREC_FP32])
+def test_noalloc_matches_transport_packets(prec):
+    box = W.box_10pc()
+    ncell, nsub = [32, 32, 32], [2, 2, 2]
+    n, xn = _medium(32, 7)
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as emitter, \
+            cmi.Context(box, ncell, nsub, prec) as ctx:
+        emitter.set_sources([[0.3e16, -1.1e16, 0.7e16], [-2.0e16, 1.0e16, 0.0]],
+                            [3e49, 1e49])
+        ctx.set_fields(n, xn)
+        # second call smaller than the first: the counters and the staged
+        # records of the first call are still in the shared array
+        for it, N in enumerate((50000, 12345)):
+            pk, ss = emitter.emit(N, 11, it)
+            print("noalloc case: precision %d, call %d, %d packets" % (prec, it, N),
+                  flush=True)
+            ref = _run(ctx, pk, ss, noalloc=False)
+            got = _run(ctx, pk, ss, noalloc=True)
+            for a, b, what in zip(ref[:4], got[:4], ("packets", "status", "crossings",
+                                                     "hops")):
+                if what == "packets":
+                    for f in ("pos", "dir", "tau"):
+                        assert np.array_equal(a[f], b[f]), (what, f)
+                else:
+                    assert np.array_equal(a, b), what
+            for k in ("packets", "crossings", "hops", "absorbed", "escaped", "errors"):
+                assert ref[4][k] == got[4][k], k
+            assert got[4]["packets"] == N
+            assert got[4]["absorbed"] + got[4]["escaped"] == N
+            assert got[4]["crossings"] == int(got[2].sum())
+            Ja, Jb = ref[5], got[5]
+            scale = np.abs(Ja).max()
+            assert scale > 0
+            # the same deposits summed in another order (atomics)
+            assert np.abs(Ja - Jb).max() <= 1e-12 * scale
+
+
+def test_noalloc_rejects_lists_beyond_its_array():
+    box = W.box_10pc()
+    with cmi.Context(box, [16, 16, 16], [1, 1, 1], cmi.PREC_FP64) as ctx:
+        N = cmi.NOALLOC_MAX_PACKETS + 1
+        pk = np.zeros(N, cmi.PHOTON_DTYPE)
+        pk["dir"][:, 0] = 1.0
+        pk["tau"] = 1.0
+        pk["sigma"] = 6.3e-22
+        pk["weight"] = 1.0
+        ss = np.zeros(N, np.uint32)
+        with pytest.raises(cmi.CMIError) as e:
+            ctx.transport_packets(pk, ss, noalloc=True)
+        assert e.value.code == -1 and "at most 65536" in str(e.value)
+        # an empty list is a no-op
+        p, st, cr, hp, stats = ctx.transport_packets(pk[:0], ss[:0], noalloc=True)
+        assert len(p) == 0 and stats["packets"] == 0
