@@ -76,6 +76,7 @@ typedef struct {
   uint32_t rounds;    /* hop rounds (tiles engine), 0 for walk */
   uint32_t reserved;
   double seconds;     /* device time of the transport (HIP events) */
+  uint64_t reemitted; /* reemission events (an absorbed packet sent on) */
 } cmi_stats;
 
 /* Tuning of the engines (0 = default for any field). */
@@ -120,9 +121,27 @@ int cmi_set_engine(cmi_ctx *ctx, int engine, const cmi_tuning *tuning);
  * buffer enabled at 1188-1191, reemission_probability of the parameter file):
  * an absorbed packet is reemitted from its absorption point with this
  * probability, with a new isotropic direction and target optical depth.
- * 0 (the default) disables it, as in the reference.  Runs in cmi_transport on
- * the tiles engine; the walk engine rejects a non-zero probability. */
+ * 0 (the default) disables it, as in the reference.  It applies to
+ * cmi_transport on both engines and to cmi_transport_packets (and its
+ * _noalloc variant) on the walk engine; cmi_transport_packets on the tiles
+ * engine refuses a non-zero probability (CMI_ERR_ARG).  The per-subgrid walks
+ * (cmi_interact_batch, cmi_propagate_batch, cmi_optical_depth_batch) are
+ * DensitySubGrid::interact: an absorbed packet comes back with
+ * TRAVELDIRECTION_INSIDE and reemitting it is the caller's step, as
+ * do_reemission is the reference driver's.
+ * Draws: the walk engine gives the r-th absorption of packet number k of
+ * cmi_transport(seed, iteration) the Philox counters {k, iteration,
+ * 2 + 256 r + j} under key seed, with draws exact in IEEE arithmetic (a
+ * uniform < p, Marsaglia's pair for the direction, von Neumann's comparison
+ * method for the optical depth); the tiles engine draws FP32 values keyed by
+ * the absorption point. */
 int cmi_set_reemission(cmi_ctx *ctx, double probability);
+
+/* Key of the reemission draws of explicit-list transports on the walk engine
+ * (default 0, 0): packet i of the list draws as packet number i of
+ * cmi_transport(seed, iteration).  No counterpart in the reference, whose
+ * threads draw from their own RANLUX generators. */
+int cmi_set_reemission_draws(cmi_ctx *ctx, uint64_t seed, uint32_t iteration);
 
 /* Copy number density and neutral fraction in (either may be NULL to keep
  * the current values).  Replaces DensitySubGrid::set_number_density and

@@ -345,17 +345,39 @@ uint32_t orc_grid_source_subgrid(const orc_grid *g, const double pos[3]) {
  * direction.  Packets are independent while x_n is frozen, so the order in
  * which the reference drains its 200-packet buffers only changes the order of
  * the J sums, not any packet's path. */
+/* Reemission of a list transport: probability, Philox key and iteration;
+ * packet i of the list draws as packet number first + i. */
+typedef struct {
+  double prob;
+  uint64_t seed;
+  uint32_t iteration;
+  uint64_t first;
+} reemit_cfg;
+
 static void transport_one(orc_grid *g, orc_photon *p, uint32_t sub,
                           int32_t *status, uint32_t *crossings,
-                          uint32_t *hops) {
+                          uint32_t *hops, const reemit_cfg *re, uint64_t k,
+                          uint32_t *reemissions) {
   int in_dir = ORC_INSIDE;
-  uint32_t nc = 0, nh = 0;
+  uint32_t nc = 0, nh = 0, nre = 0;
   for (;;) {
     orc_subgrid *s = &g->sub[sub];
     ++nh;
     const int out = orc_interact(s, p, in_dir, &nc);
     if (out == ORC_INSIDE) {
-      *status = 0;
+      /* do_reemission (testDensitySubGrid_omp.cpp:675-702): the absorbed
+       * packet sits in its subgrid's INSIDE buffer (1188-1191) and is
+       * reemitted with the given probability from where it stopped (its
+       * position stays), then traversed again in the same subgrid with
+       * input direction INSIDE (execute_photon_reemit_task, 1348-) */
+      if (re && re->prob > 0. && nre < ORC_REEMIT_MAX &&
+          orc_reemit_draw(k, nre, re->prob, re->seed, re->iteration, p->dir,
+                          &p->tau)) {
+        ++nre;
+        in_dir = ORC_INSIDE;
+        continue;
+      }
+      *status = (re && re->prob > 0. && nre >= ORC_REEMIT_MAX) ? -2 : 0;
       break;
     }
     if (out < 0) {
@@ -372,19 +394,49 @@ static void transport_one(orc_grid *g, orc_photon *p, uint32_t sub,
   }
   if (crossings) *crossings = nc;
   if (hops) *hops = nh;
+  if (reemissions) *reemissions = nre;
 }
+
+static void transport_list_impl(orc_grid *g, orc_photon *p,
+                                const uint32_t *start_sub, size_t n,
+                                int32_t *status, uint32_t *crossings,
+                                uint32_t *hops, uint32_t *reemissions,
+                                const reemit_cfg *re, int nthreads);
 
 void orc_transport_list(orc_grid *g, orc_photon *p, const uint32_t *start_sub,
                         size_t n, int32_t *status, uint32_t *crossings,
                         uint32_t *hops, int nthreads) {
+  transport_list_impl(g, p, start_sub, n, status, crossings, hops, NULL, NULL,
+                      nthreads);
+}
+
+void orc_transport_list_reemit(orc_grid *g, orc_photon *p,
+                               const uint32_t *start_sub, size_t n,
+                               double probability, uint64_t seed,
+                               uint32_t iteration, uint64_t first,
+                               int32_t *status, uint32_t *crossings,
+                               uint32_t *hops, uint32_t *reemissions,
+                               int nthreads) {
+  const reemit_cfg re = {probability, seed, iteration, first};
+  transport_list_impl(g, p, start_sub, n, status, crossings, hops, reemissions,
+                      &re, nthreads);
+}
+
+static void transport_list_impl(orc_grid *g, orc_photon *p,
+                                const uint32_t *start_sub, size_t n,
+                                int32_t *status, uint32_t *crossings,
+                                uint32_t *hops, uint32_t *reemissions,
+                                const reemit_cfg *re, int nthreads) {
+  const uint64_t first = re ? re->first : 0;
   if (nthreads <= 1) {
     for (size_t i = 0; i < n; ++i) {
       int32_t st;
-      uint32_t c, h;
-      transport_one(g, &p[i], start_sub[i], &st, &c, &h);
+      uint32_t c, h, r;
+      transport_one(g, &p[i], start_sub[i], &st, &c, &h, re, first + i, &r);
       if (status) status[i] = st;
       if (crossings) crossings[i] = c;
       if (hops) hops[i] = h;
+      if (reemissions) reemissions[i] = r;
     }
     return;
   }
@@ -411,11 +463,13 @@ void orc_transport_list(orc_grid *g, orc_photon *p, const uint32_t *start_sub,
 #pragma omp for schedule(dynamic, 256)
     for (long long i = 0; i < (long long)n; ++i) {
       int32_t st;
-      uint32_t c, h;
-      transport_one(&local, &p[i], start_sub[i], &st, &c, &h);
+      uint32_t c, h, r;
+      transport_one(&local, &p[i], start_sub[i], &st, &c, &h, re,
+                    first + (uint64_t)i, &r);
       if (status) status[i] = st;
       if (crossings) crossings[i] = c;
       if (hops) hops[i] = h;
+      if (reemissions) reemissions[i] = r;
     }
     free(subs);
   }
@@ -425,7 +479,8 @@ void orc_transport_list(orc_grid *g, orc_photon *p, const uint32_t *start_sub,
   }
   free(priv);
 #else
-  orc_transport_list(g, p, start_sub, n, status, crossings, hops, 1);
+  transport_list_impl(g, p, start_sub, n, status, crossings, hops, reemissions,
+                      re, 1);
 #endif
 }
 
@@ -496,6 +551,87 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
 double orc_uniform_open(uint32_t hi, uint32_t lo) {
   const uint64_t m = ((uint64_t)(hi >> 5) << 26) | (uint64_t)(lo >> 6);
   return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+/* Reemission draws of packet k's r-th absorption (do_reemission,
+ * testDensitySubGrid_omp.cpp:675-702: a uniform below the probability, then
+ * get_random_direction 529-545 and tau = -ln u).  The draws use only
+ * operations IEEE 754 rounds exactly (+ - * / sqrt and compares), so that
+ * this restatement and the GPU walk give the same bits:
+ *   - the direction by Marsaglia's rejection pair (Ann. Math. Stat. 43
+ *     (1972) 645): a, b uniform in (-1, 1) with s = a^2 + b^2 < 1, then
+ *     (2a sqrt(1 - s), 2b sqrt(1 - s), 1 - 2s) is uniform on the sphere;
+ *   - the optical depth by von Neumann's comparison method (NBS Appl. Math.
+ *     Ser. 12 (1951) 36; Knuth, TAOCP 2, 3.4.1): trial k draws U0 > U1 > ...
+ *     until a draw exceeds its predecessor; an odd run length accepts
+ *     k + U0, which is then Exp(1), like -ln u.
+ * The uniforms come in order from Philox counters {k_lo, k_hi, iteration,
+ * 2 + 256 r + j}, j = 0, 1, ..., two per counter (the emission uses 0 and
+ * 1).  Returns 1 and writes dir and tau when the packet is reemitted. */
+typedef struct {
+  uint32_t key[2], ctr[4], buf[4];
+  int avail, blocks;
+} reemit_stream;
+
+static double rs_uniform(reemit_stream *s) {
+  if (s->avail == 0) {
+    orc_philox4x32_10(s->ctr, s->key, s->buf);
+    s->ctr[3] += 1u;
+    s->blocks += 1;
+    s->avail = 2;
+  }
+  const int at = 2 - s->avail;
+  s->avail -= 1;
+  return orc_uniform_open(s->buf[2 * at], s->buf[2 * at + 1]);
+}
+
+int orc_reemit_draw(uint64_t k, uint32_t r, double probability, uint64_t seed,
+                    uint32_t iteration, double dir[3], double *tau) {
+  reemit_stream s;
+  s.key[0] = (uint32_t)seed;
+  s.key[1] = (uint32_t)(seed >> 32);
+  s.ctr[0] = (uint32_t)k;
+  s.ctr[1] = (uint32_t)(k >> 32);
+  s.ctr[2] = iteration;
+  s.ctr[3] = 2u + 256u * r;
+  s.avail = 0;
+  s.blocks = 0;
+  if (!(rs_uniform(&s) < probability)) return 0;
+  dir[0] = 0.;
+  dir[1] = 0.;
+  dir[2] = 1.;
+  for (int t = 0; t < ORC_REEMIT_DIR_TRIES; ++t) {
+    const double a = 2. * rs_uniform(&s) - 1.;
+    const double b = 2. * rs_uniform(&s) - 1.;
+    const double q = a * a + b * b;
+    if (q < 1. && q > 0.) {
+      const double f = 2. * sqrt(1. - q);
+      dir[0] = a * f;
+      dir[1] = b * f;
+      dir[2] = 1. - 2. * q;
+      break;
+    }
+  }
+  double trials = 0.;
+  *tau = 0.5;
+  while (s.blocks < ORC_REEMIT_BLOCKS) {
+    const double u0 = rs_uniform(&s);
+    double prev = u0;
+    int run = 1;
+    for (;;) {
+      const double v = rs_uniform(&s);
+      if (!(v < prev)) break;
+      prev = v;
+      ++run;
+      if (s.blocks >= ORC_REEMIT_BLOCKS) break; /* bound (never reached) */
+    }
+    if (run & 1) {
+      *tau = trials + u0;
+      break;
+    }
+    trials += 1.;
+  }
+  return 1;
 }
 
 /* testDensitySubGrid_omp.cpp:2199-2208 and 2240-2273 (with distribute(),

@@ -106,6 +106,29 @@ void orc_transport_list(orc_grid *g, orc_photon *p, const uint32_t *start_sub,
                         size_t n, int32_t *status, uint32_t *crossings,
                         uint32_t *hops, int nthreads);
 
+/* The same with reemission (do_reemission, testDensitySubGrid_omp.cpp:
+ * 675-702): an absorbed packet is reemitted with the given probability from
+ * its absorption point, in the same subgrid, entering INSIDE.  Packet i of
+ * the list draws as packet number first + i of (seed, iteration), see
+ * orc_reemit_draw.  status -2: ORC_REEMIT_MAX reemissions reached.
+ * reemissions (may be NULL) receives each packet's count. */
+#define ORC_REEMIT_MAX 65536
+void orc_transport_list_reemit(orc_grid *g, orc_photon *p,
+                               const uint32_t *start_sub, size_t n,
+                               double probability, uint64_t seed,
+                               uint32_t iteration, uint64_t first,
+                               int32_t *status, uint32_t *crossings,
+                               uint32_t *hops, uint32_t *reemissions,
+                               int nthreads);
+
+/* Draws of packet k's r-th absorption: 1 (and a new direction and target
+ * optical depth) if it is reemitted, 0 if not.  At most ORC_REEMIT_BLOCKS
+ * Philox blocks per absorption and ORC_REEMIT_DIR_TRIES direction pairs. */
+#define ORC_REEMIT_BLOCKS 256
+#define ORC_REEMIT_DIR_TRIES 32
+int orc_reemit_draw(uint64_t k, uint32_t r, double probability, uint64_t seed,
+                    uint32_t iteration, double dir[3], double *tau);
+
 /* ---- ionization (DensitySubGrid.hpp:1628-1658) ---- */
 /* form 0: the reference expression 1 + aa(1 - sqrt(2/aa + 1));
  * form 1: the cancellation-free root 2 / (2 + b + sqrt(b(b + 4))), b = 2 aa. */

@@ -49,7 +49,7 @@ class Stats(C.Structure):
                 ("hops", C.c_uint64), ("absorbed", C.c_uint64),
                 ("escaped", C.c_uint64), ("errors", C.c_uint64),
                 ("rounds", C.c_uint32), ("reserved", C.c_uint32),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("reemitted", C.c_uint64)]
 
 
 STATS_FIELDS = [f for f, _ in Stats._fields_ if f != "reserved"]
@@ -77,6 +77,7 @@ SIGNATURES = {
     "cmi_last_error": (C.c_char_p, [_vp]),
     "cmi_set_engine": (C.c_int, [_vp, C.c_int, C.POINTER(Tuning)]),
     "cmi_set_reemission": (C.c_int, [_vp, C.c_double]),
+    "cmi_set_reemission_draws": (C.c_int, [_vp, C.c_uint64, C.c_uint32]),
     "cmi_set_fields": (C.c_int, [_vp, _dp, _dp]),
     "cmi_set_intensity": (C.c_int, [_vp, _dp]),
     "cmi_get_fields": (C.c_int, [_vp, _dp, _dp]),
@@ -241,8 +242,14 @@ class Context:
         t.lds_fp32 = int(lds_fp32)
         self._check(self.lib.cmi_set_engine(self.h, int(engine), C.byref(t)))
 
-    def set_reemission(self, probability):
+    def set_reemission(self, probability, seed=None, iteration=0):
+        """Reemission probability; seed/iteration (if given) key the draws of
+        explicit-list transports on the walk engine (packet i of a list draws
+        as packet number i of transport(seed, iteration))."""
         self._check(self.lib.cmi_set_reemission(self.h, float(probability)))
+        if seed is not None:
+            self._check(self.lib.cmi_set_reemission_draws(self.h, int(seed),
+                                                          int(iteration)))
 
     def set_fields(self, nH=None, xn=None):
         a = None if nH is None else np.ascontiguousarray(nH, dtype=np.float64)

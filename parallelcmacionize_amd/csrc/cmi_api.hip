@@ -145,6 +145,7 @@ int ensure_sigma(cmi_ctx *c, double sigma) {
 void fill_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets, double seconds,
                 uint32_t rounds) {
   if (!st) return;
+  st->reemitted = c->h_cnt->reemitted;
   st->packets = packets;
   st->crossings = c->h_cnt->crossings;
   st->hops = c->h_cnt->hops;
@@ -181,6 +182,15 @@ int prepare_list_transport(cmi_ctx *c, const cmi_photon *p,
 void fill_ctx_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets,
                     double seconds) {
   fill_stats(c, st, packets, seconds, 0);
+}
+
+ReemitDraws list_reemit_draws(const cmi_ctx *c) {
+  ReemitDraws re{};
+  re.p = c->reemission;
+  re.key0 = (uint32_t)c->reemit_seed;
+  re.key1 = (uint32_t)(c->reemit_seed >> 32);
+  re.iteration = c->reemit_iteration;
+  return re;
 }
 
 // n_H of a context in the reference layout (for the result writers)
@@ -527,9 +537,13 @@ int cmi_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub,
     if (stats) memset(stats, 0, sizeof(*stats));
     return CMI_OK;
   }
-  if (c->engine == CMI_ENGINE_TILES)
+  if (c->engine == CMI_ENGINE_TILES) {
+    if (c->reemission > 0.)
+      return fail(c, CMI_ERR_ARG,
+                  "reemission on explicit packet lists runs on the walk engine");
     return tiles_transport_packets(c, p, start_sub, n, status, crossings, hops,
                                    stats);
+  }
   PhotonRec *dp = nullptr;
   uint32_t *ds = nullptr, *dc = nullptr, *dh = nullptr;
   int32_t *dst = nullptr;
@@ -546,7 +560,7 @@ int cmi_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub,
   if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
   if (e == hipSuccess)
     e = launch_walk_list(c->g, c->f, c->precision, dp, ds, n, dst, dc, dh,
-                         c->d_cnt, c->stream);
+                         list_reemit_draws(c), c->d_cnt, c->stream);
   if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(p, dp, n * sizeof(PhotonRec),
                                           hipMemcpyDeviceToHost, c->stream);
@@ -631,6 +645,13 @@ int cmi_set_reemission(cmi_ctx *c, double probability) {
   return CMI_OK;
 }
 
+int cmi_set_reemission_draws(cmi_ctx *c, uint64_t seed, uint32_t iteration) {
+  if (!c) return CMI_ERR_ARG;
+  c->reemit_seed = seed;
+  c->reemit_iteration = iteration;
+  return CMI_OK;
+}
+
 int cmi_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
                   uint32_t iteration, double sigma, cmi_stats *stats) {
   if (!c) return CMI_ERR_ARG;
@@ -641,14 +662,17 @@ int cmi_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   if (rc != CMI_OK) return rc;
   if (c->engine == CMI_ENGINE_TILES)
     return tiles_transport(c, npackets, seed, iteration, sigma, stats);
-  if (c->reemission > 0.)
-    return fail(c, CMI_ERR_ARG, "reemission runs on the tiles engine");
+  ReemitDraws re{};
+  re.p = c->reemission;
+  re.key0 = (uint32_t)seed;
+  re.key1 = (uint32_t)(seed >> 32);
+  re.iteration = iteration;
   const int blocks = c->tuning.walk_blocks > 0 ? c->tuning.walk_blocks
                                                : c->num_cus * 8;
   CMI_TRY_HIP(c, hipMemsetAsync(c->d_cnt, 0, sizeof(Counters), c->stream));
   CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
   CMI_TRY_HIP(c, launch_walk_emit(c->g, c->f, c->precision, c->d_src, npackets,
-                                  iteration, seed, sigma, blocks, c->d_cnt,
+                                  iteration, seed, sigma, blocks, re, c->d_cnt,
                                   c->stream));
   CMI_TRY_HIP(c, hipEventRecord(c->ev1, c->stream));
   CMI_TRY_HIP(c, hipMemcpyAsync(c->h_cnt, c->d_cnt, sizeof(Counters),

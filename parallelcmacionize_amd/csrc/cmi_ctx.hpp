@@ -37,7 +37,9 @@ struct cmi_ctx {
   std::vector<double> h_partial;
   std::vector<double> src_pos, src_L;
   double sigma = 6.3e-22;  // cross section folded into kappa (FP32 mode)
-  double reemission = 0.;  // reemission probability (tiles engine)
+  double reemission = 0.;  // reemission probability
+  uint64_t reemit_seed = 0;       // key of the list transports' reemission draws
+  uint32_t reemit_iteration = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double kernel_time_sum = 0.;
   uint64_t kernel_calls = 0;
@@ -107,6 +109,11 @@ int prepare_list_transport(cmi_ctx *c, const cmi_photon *p,
                            const uint32_t *start_sub, size_t n);
 void fill_ctx_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets,
                     double seconds);
+
+// Reemission draws of the walk engine: list transports (key of
+// cmi_set_reemission_draws) and emitted ones (the transport's seed and
+// iteration).
+ReemitDraws list_reemit_draws(const cmi_ctx *c);
 
 // Device address of the allocation-free entry point's scratch array on the
 // context's device (cmi_noalloc.hip); resolved once per device.

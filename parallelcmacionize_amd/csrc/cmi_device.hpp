@@ -173,7 +173,89 @@ struct Counters {
   unsigned long long escaped;
   unsigned long long errors;  // invalid exit mask or exceeded walk bound
   unsigned long long next_packet;  // persistent-refill ticket
-  unsigned long long pad[2];
+  unsigned long long reemitted;    // reemission events
+  unsigned long long pad;
 };
+
+// ---------------------------------------------------------------------------
+// Reemission draws (do_reemission, testDensitySubGrid_omp.cpp:675-702: a
+// uniform below the probability, then get_random_direction 529-545 and
+// tau = -ln u) for the r-th absorption of packet k.  Only operations IEEE 754
+// rounds exactly (+ - * / sqrt and compares), so that a CPU restatement gives
+// the same bits:
+//   - the direction by Marsaglia's rejection pair (Ann. Math. Stat. 43 (1972)
+//     645): a, b uniform in (-1, 1) with q = a^2 + b^2 < 1, then
+//     (2a sqrt(1 - q), 2b sqrt(1 - q), 1 - 2q) is uniform on the sphere;
+//   - the optical depth by von Neumann's comparison method (NBS Appl. Math.
+//     Ser. 12 (1951) 36; Knuth, TAOCP 2, 3.4.1): trial j draws U0 > U1 > ...
+//     until a draw exceeds its predecessor; an odd run length accepts
+//     j + U0, which is Exp(1) like -ln u.
+// Uniforms are taken in order, two per Philox block, from counters
+// {k_lo, k_hi, iteration, 2 + 256 r + b}, b = 0, 1, ... (emission uses 0, 1).
+// ---------------------------------------------------------------------------
+constexpr int kReemitBlocks = 256;       // Philox blocks per absorption, at most
+constexpr int kReemitDirTries = 32;      // direction pairs, at most
+constexpr uint32_t kReemitMax = 65536;   // reemissions per packet, at most
+
+struct ReemitDraws {
+  double p;  // probability (0: off)
+  uint32_t key0, key1, iteration, pad;
+};
+
+__host__ __device__ inline bool reemit_draw(uint64_t k, uint32_t r,
+                                            const ReemitDraws &R, double dir[3],
+                                            double *tau) {
+#pragma clang fp contract(off)
+  u32x4 ctr{(uint32_t)k, (uint32_t)(k >> 32), R.iteration, 2u + 256u * r};
+  u32x4 buf{0u, 0u, 0u, 0u};
+  int avail = 0, blocks = 0;
+  auto uniform = [&]() -> double {
+    if (avail == 0) {
+      buf = philox4x32_10(ctr, R.key0, R.key1);
+      ctr.w += 1u;
+      blocks += 1;
+      avail = 2;
+    }
+    const bool first_pair = avail == 2;
+    avail -= 1;
+    return first_pair ? uniform_open(buf.x, buf.y) : uniform_open(buf.z, buf.w);
+  };
+  if (!(uniform() < R.p)) return false;
+  dir[0] = 0.;
+  dir[1] = 0.;
+  dir[2] = 1.;
+  for (int t = 0; t < kReemitDirTries; ++t) {
+    const double a = 2. * uniform() - 1.;
+    const double b = 2. * uniform() - 1.;
+    const double q = a * a + b * b;
+    if (q < 1. && q > 0.) {
+      const double f = 2. * sqrt(1. - q);
+      dir[0] = a * f;
+      dir[1] = b * f;
+      dir[2] = 1. - 2. * q;
+      break;
+    }
+  }
+  double trials = 0.;
+  *tau = 0.5;
+  while (blocks < kReemitBlocks) {
+    const double u0 = uniform();
+    double prev = u0;
+    int run = 1;
+    for (;;) {
+      const double v = uniform();
+      if (!(v < prev)) break;
+      prev = v;
+      ++run;
+      if (blocks >= kReemitBlocks) break;  // bound (never reached)
+    }
+    if (run & 1) {
+      *tau = trials + u0;
+      break;
+    }
+    trials += 1.;
+  }
+  return true;
+}
 
 }  // namespace cmi

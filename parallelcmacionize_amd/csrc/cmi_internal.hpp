@@ -34,11 +34,13 @@ hipError_t launch_interact_batch(const GridDev &g, const double *n,
 hipError_t launch_walk_list(const GridDev &g, const FieldsDev &f, int prec,
                             PhotonRec *p, const uint32_t *start_sub, size_t np,
                             int32_t *status, uint32_t *crossings,
-                            uint32_t *hops, Counters *cnt, hipStream_t st);
+                            uint32_t *hops, const ReemitDraws &re,
+                            Counters *cnt, hipStream_t st);
 hipError_t launch_walk_emit(const GridDev &g, const FieldsDev &f, int prec,
                             const SourcesDev *S, uint64_t npackets,
                             uint32_t iteration, uint64_t seed, double sigma,
-                            int blocks, Counters *cnt, hipStream_t st);
+                            int blocks, const ReemitDraws &re, Counters *cnt,
+                            hipStream_t st);
 hipError_t launch_emit(const SourcesDev *S, uint64_t first, uint64_t count,
                        uint32_t iteration, uint64_t seed, double sigma,
                        PhotonRec *out, uint32_t *start_sub, hipStream_t st);

@@ -108,7 +108,7 @@ int cmi_transport_packets_noalloc(cmi_ctx *c, cmi_photon *p,
                              sizeof(Counters) / sizeof(double), 0., c->stream));
   CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
   CMI_TRY_HIP(c, launch_walk_list(c->g, c->f, c->precision, dp, ds, n, dst, dc,
-                                  dh, cnt, c->stream));
+                                  dh, list_reemit_draws(c), cnt, c->stream));
   CMI_TRY_HIP(c, hipEventRecord(c->ev1, c->stream));
   CMI_TRY_HIP(c, hipMemcpyAsync(p, dp, n * sizeof(PhotonRec),
                                 hipMemcpyDeviceToHost, c->stream));

@@ -633,6 +633,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   // per-lane counters of this launch (32 bits: a lane walks at most a few
   // hundred thousand crossings per launch); widened for the final sums
   uint32_t my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0, my_err = 0;
+  uint32_t my_re = 0;  // reemissions (kReemit only)
 
   // a workgroup's first visit is the descriptor of its own index; later ones
   // come from the ticket, counted on from the grid size (no start-of-launch
@@ -872,6 +873,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             p2 = q2;
             t = 0.f;
             ++n_reemit;
+            ++my_re;
             steps = 0;
             const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
                         i2 = __builtin_amdgcn_rcpf(d2);
@@ -992,6 +994,12 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     if (w_abs) atomicAdd(&cnt->absorbed, w_abs);
     if (w_esc) atomicAdd(&cnt->escaped, w_esc);
     if (w_err) atomicAdd(&cnt->errors, w_err);
+  }
+  if (kReemit) {
+    unsigned long long w_re = my_re;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) w_re += __shfl_xor(w_re, o, 64);
+    if (lane == 0 && w_re) atomicAdd(&cnt->reemitted, w_re);
   }
 }
 
@@ -1451,6 +1459,7 @@ static void fill_set_sources_single(cmi_ctx *c, TilesEngine *e, uint64_t seed,
 static void fill_stats_tiles(cmi_ctx *c, cmi_stats *stats, uint64_t n,
                              uint32_t rounds, double seconds) {
   if (!stats) return;
+  stats->reemitted = c->h_cnt->reemitted;
   stats->packets = n;
   stats->crossings = c->h_cnt->crossings;
   stats->hops = c->h_cnt->hops;
@@ -1516,6 +1525,9 @@ int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub
   if (c->precision != CMI_PREC_FP32)
     return fail(c, CMI_ERR_ARG, "the tiles engine runs FP32 contexts");
   if (n > 0xfffffff0ull) return fail(c, CMI_ERR_ARG, "too many packets");
+  if (c->reemission > 0.)
+    return fail(c, CMI_ERR_ARG,
+                "reemission on explicit packet lists runs on the walk engine");
   int rc = ensure_engine(c, ctx_problems(c), n, true);
   if (rc != CMI_OK) return rc;
   TilesEngine *e = c->tiles;

@@ -339,23 +339,43 @@ __global__ void interact_batch_kernel(GridDev g, const double *n,
 // Kernel: whole-grid transport of an explicit packet list (status 0 absorbed,
 // 1 escaped, -1 invalid exit, -2 walk bound).  One thread per packet.
 // ---------------------------------------------------------------------------
+// Reemission of an absorbed packet (do_reemission and the reemit task,
+// testDensitySubGrid_omp.cpp:675-702, 1348-): with probability re.p the
+// packet gets the draws of its nre-th absorption and is walked again through
+// the same subgrid from where it stopped, entering INSIDE (its absolute
+// position, as the reference keeps it between the tasks).  Returns true if
+// the packet was reemitted; the new direction also goes to dir_out.
+template <class T>
+__device__ __forceinline__ bool reemit(Walker<T> &w, const GridDev &g,
+                                       const ReemitDraws &re, uint64_t k,
+                                       uint32_t nre, double dir_out[3]) {
+  double pa[3], dir[3], tau;
+  if (!reemit_draw(k, nre, re, dir, &tau)) return false;
+  abs_position(w, g, pa);
+  load_packet(w, g, w.sub, pa, dir, tau, TD_INSIDE);
+  dir_out[0] = dir[0];
+  dir_out[1] = dir[1];
+  dir_out[2] = dir[2];
+  return true;
+}
+
 template <class Mode>
 __global__ void walk_list_kernel(GridDev g, Mode m, PhotonRec *p,
                                  const uint32_t *start_sub, size_t np,
                                  int32_t *status, uint32_t *crossings,
                                  uint32_t *hops, int max_steps, int max_hops,
-                                 Counters *cnt) {
+                                 ReemitDraws re, Counters *cnt) {
   using T = typename Mode::real;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0,
-                     my_err = 0;
+                     my_err = 0, my_re = 0;
   if (i < np) {
     PhotonRec ph = p[i];
     Walker<T> w;
     load_packet(w, g, (int)start_sub[i], ph.pos, ph.dir, ph.tau, TD_INSIDE);
-    uint32_t nc = 0, nh = 1;
+    uint32_t nc = 0, nh = 1, nre = 0;
     int st = -2;
-    int steps = 0;
+    int steps = 0, leg_hops = 1;
     for (;;) {
       int r = visit_open(w, g);
       while (r == kContinue && steps < max_steps) {
@@ -369,6 +389,17 @@ __global__ void walk_list_kernel(GridDev g, Mode m, PhotonRec *p,
       }
       if (r == kAbsorbed) {
         st = 0;
+        if (re.p > 0.) {
+          if (nre >= kReemitMax) {
+            st = -2;
+          } else if (reemit(w, g, re, (uint64_t)i, nre, ph.dir)) {
+            ++nre;
+            ++nh;
+            steps = 0;
+            leg_hops = 1;
+            continue;
+          }
+        }
         break;
       }
       const int h = hop(w, g);
@@ -377,7 +408,8 @@ __global__ void walk_list_kernel(GridDev g, Mode m, PhotonRec *p,
         break;
       }
       steps = 0;
-      if ((int)++nh > max_hops) {
+      ++nh;
+      if (++leg_hops > max_hops) {
         st = -2;
         break;
       }
@@ -393,12 +425,14 @@ __global__ void walk_list_kernel(GridDev g, Mode m, PhotonRec *p,
     my_abs = (st == 0);
     my_esc = (st == 1);
     my_err = (st < 0);
+    my_re = nre;
   }
   wave_add(&cnt->crossings, my_cross);
   wave_add(&cnt->hops, my_hops);
   wave_add(&cnt->absorbed, my_abs);
   wave_add(&cnt->escaped, my_esc);
   wave_add(&cnt->errors, my_err);
+  wave_add(&cnt->reemitted, my_re);
 }
 
 // ---------------------------------------------------------------------------
@@ -411,15 +445,17 @@ template <class Mode>
 __global__ void __launch_bounds__(256) walk_emit_kernel(
     GridDev g, Mode m, const SourcesDev *__restrict__ S, uint64_t npackets,
     uint32_t iteration, uint32_t key0, uint32_t key1, double sigma,
-    int max_steps, int max_hops, Counters *cnt) {
+    int max_steps, int max_hops, ReemitDraws re, Counters *cnt) {
   using T = typename Mode::real;
   const int lane = threadIdx.x & 63;
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   Walker<T> w;
   bool live = false, exhausted = false;
   int steps = 0, nh = 0;
+  uint64_t my_k = 0;  // packet number of the lane's packet
+  uint32_t nre = 0;   // its reemissions so far
   unsigned long long my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0,
-                     my_err = 0;
+                     my_err = 0, my_re = 0;
   for (;;) {
     const bool need = !live && !exhausted;
     const uint64_t want = __ballot(need);
@@ -439,6 +475,8 @@ __global__ void __launch_bounds__(256) walk_emit_kernel(
           live = true;
           steps = 0;
           nh = 1;
+          my_k = k;
+          nre = 0;
           ++my_hops;
         } else {
           exhausted = true;
@@ -461,6 +499,22 @@ __global__ void __launch_bounds__(256) walk_emit_kernel(
     }
     if (r == kContinue) continue;
     if (r == kAbsorbed) {
+      if (re.p > 0.) {
+        double dir_unused[3];
+        if (nre >= kReemitMax) {
+          ++my_err;
+          live = false;
+          continue;
+        }
+        if (reemit(w, g, re, my_k, nre, dir_unused)) {
+          ++nre;
+          ++my_re;
+          ++my_hops;
+          steps = 0;
+          nh = 1;
+          continue;
+        }
+      }
       ++my_abs;
       live = false;
       continue;
@@ -483,6 +537,7 @@ __global__ void __launch_bounds__(256) walk_emit_kernel(
   wave_add(&cnt->absorbed, my_abs);
   wave_add(&cnt->escaped, my_esc);
   wave_add(&cnt->errors, my_err);
+  wave_add(&cnt->reemitted, my_re);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,44 +585,46 @@ template <class Mode>
 static hipError_t launch_walk_list_t(const GridDev &g, Mode m, PhotonRec *p,
                                      const uint32_t *start_sub, size_t np,
                                      int32_t *status, uint32_t *crossings,
-                                     uint32_t *hops, Counters *cnt,
-                                     hipStream_t st) {
+                                     uint32_t *hops, const ReemitDraws &re,
+                                     Counters *cnt, hipStream_t st) {
   if (np == 0) return hipSuccess;
   const int max_steps = g.sc[0] + g.sc[1] + g.sc[2] + 8;
   const int max_hops = 4 * (g.nsub[0] + g.nsub[1] + g.nsub[2]) + 16;
   const unsigned blocks = (unsigned)((np + 255) / 256);
   hipLaunchKernelGGL(walk_list_kernel<Mode>, dim3(blocks), dim3(256), 0, st, g,
                      m, p, start_sub, np, status, crossings, hops, max_steps,
-                     max_hops, cnt);
+                     max_hops, re, cnt);
   return hipGetLastError();
 }
 
 hipError_t launch_walk_list(const GridDev &g, const FieldsDev &f, int prec,
                             PhotonRec *p, const uint32_t *start_sub, size_t np,
                             int32_t *status, uint32_t *crossings,
-                            uint32_t *hops, Counters *cnt, hipStream_t st) {
+                            uint32_t *hops, const ReemitDraws &re,
+                            Counters *cnt, hipStream_t st) {
   if (prec == 0)
     return launch_walk_list_t(g, Mode64{f.n, f.xn, f.J}, p, start_sub, np,
-                              status, crossings, hops, cnt, st);
+                              status, crossings, hops, re, cnt, st);
   return launch_walk_list_t(g, Mode32{f.kappa, f.J}, p, start_sub, np, status,
-                            crossings, hops, cnt, st);
+                            crossings, hops, re, cnt, st);
 }
 
 hipError_t launch_walk_emit(const GridDev &g, const FieldsDev &f, int prec,
                             const SourcesDev *S, uint64_t npackets,
                             uint32_t iteration, uint64_t seed, double sigma,
-                            int blocks, Counters *cnt, hipStream_t st) {
+                            int blocks, const ReemitDraws &re, Counters *cnt,
+                            hipStream_t st) {
   const int max_steps = g.sc[0] + g.sc[1] + g.sc[2] + 8;
   const int max_hops = 4 * (g.nsub[0] + g.nsub[1] + g.nsub[2]) + 16;
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   if (prec == 0) {
     hipLaunchKernelGGL(walk_emit_kernel<Mode64>, dim3(blocks), dim3(256), 0,
                        st, g, Mode64{f.n, f.xn, f.J}, S, npackets, iteration,
-                       k0, k1, sigma, max_steps, max_hops, cnt);
+                       k0, k1, sigma, max_steps, max_hops, re, cnt);
   } else {
     hipLaunchKernelGGL(walk_emit_kernel<Mode32>, dim3(blocks), dim3(256), 0,
                        st, g, Mode32{f.kappa, f.J}, S, npackets, iteration,
-                       k0, k1, sigma, max_steps, max_hops, cnt);
+                       k0, k1, sigma, max_steps, max_hops, re, cnt);
   }
   return hipGetLastError();
 }

@@ -281,9 +281,16 @@ class DensitySubGridGrid {
   }
 
   // reemission_probability of the reference's parameter file (its
-  // do_reemission, testDensitySubGrid_omp.cpp:675-702); tiles engine
+  // do_reemission, testDensitySubGrid_omp.cpp:675-702): shoot() on both
+  // engines, explicit lists on the walk engine
   void set_reemission_probability(double p) {
     check(cmi_set_reemission(ctx_, p), "cmi_set_reemission");
+  }
+  // key of the reemission draws of explicit-list transports (no counterpart
+  // in the reference, whose threads own RANLUX generators)
+  void set_reemission_draws(uint64_t seed, uint32_t iteration) {
+    check(cmi_set_reemission_draws(ctx_, seed, iteration),
+          "cmi_set_reemission_draws");
   }
   // output_neutral_fractions (testDensitySubGrid_omp.cpp:394-440) and
   // output_neutral_fractions_binary (507-521): the reference's result files

@@ -67,6 +67,13 @@ def load_oracle():
     lib.orc_transport_list.argtypes = [C.POINTER(OrcGrid), C.c_void_p, _u32p,
                                        C.c_size_t, _i32p, _u32p, _u32p,
                                        C.c_int]
+    lib.orc_transport_list_reemit.argtypes = [
+        C.POINTER(OrcGrid), C.c_void_p, _u32p, C.c_size_t, C.c_double,
+        C.c_uint64, C.c_uint32, C.c_uint64, _i32p, _u32p, _u32p, _u32p,
+        C.c_int]
+    lib.orc_reemit_draw.argtypes = [C.c_uint64, C.c_uint32, C.c_double,
+                                    C.c_uint64, C.c_uint32, _dp, _dp]
+    lib.orc_reemit_draw.restype = C.c_int
     lib.orc_compute_neutral_fraction.argtypes = [C.POINTER(OrcGrid), C.c_double,
                                                  C.c_double, C.c_double,
                                                  C.c_int]
@@ -155,6 +162,23 @@ class OracleGrid:
                                     ptr(cr, _u32p), ptr(hp, _u32p), nthreads)
         return st, cr, hp
 
+    def transport_reemit(self, photons, start_sub, probability, seed,
+                         iteration, first=0, nthreads=1):
+        """Transport with reemission (orc_transport_list_reemit); returns
+        status, crossings, hops and reemissions per packet."""
+        n = len(photons)
+        st = np.zeros(n, np.int32)
+        cr = np.zeros(n, np.uint32)
+        hp = np.zeros(n, np.uint32)
+        re = np.zeros(n, np.uint32)
+        ss = np.ascontiguousarray(start_sub, dtype=np.uint32)
+        self.lib.orc_transport_list_reemit(
+            C.byref(self.g), photons.ctypes.data, ptr(ss, _u32p), n,
+            float(probability), int(seed), int(iteration), int(first),
+            ptr(st, _i32p), ptr(cr, _u32p), ptr(hp, _u32p), ptr(re, _u32p),
+            nthreads)
+        return st, cr, hp, re
+
     def compute_neutral_fraction(self, L, N, alpha, form=0):
         self.lib.orc_compute_neutral_fraction(C.byref(self.g), L, float(N),
                                               alpha, form)
@@ -182,6 +206,24 @@ class OracleGrid:
                                     iteration, sigma, alpha, form, nthreads,
                                     C.byref(l1))
         return nc, l1.value
+
+
+def reemit_draws(lib, ks, r, probability, seed, iteration):
+    """orc_reemit_draw for packet numbers ks at absorption r: reemitted flag,
+    directions and target optical depths."""
+    ks = np.asarray(ks, dtype=np.uint64)
+    ok = np.zeros(len(ks), bool)
+    dirs = np.zeros((len(ks), 3))
+    taus = np.zeros(len(ks))
+    d = np.zeros(3)
+    t = C.c_double()
+    for j, k in enumerate(ks):
+        ok[j] = lib.orc_reemit_draw(int(k), int(r), float(probability), int(seed),
+                                    int(iteration), ptr(d, _dp), C.byref(t)) == 1
+        if ok[j]:
+            dirs[j] = d
+            taus[j] = t.value
+    return ok, dirs, taus
 
 
 def make_subgrid(lib, box, ncell):

@@ -25,8 +25,9 @@ def _run(ctx, pk, ss, noalloc):
     return p, st, cr, hp, stats, J
 
 
-@pytest.mark.parametrize("prec", [cmi.PREC_FP64, cmi.PREC_FP32])
-def test_noalloc_matches_transport_packets(prec):
+@pytest.mark.parametrize("prec,reemission", [(cmi.PREC_FP64, 0.0), (cmi.PREC_FP32, 0.0),
+                                             (cmi.PREC_FP64, 0.5)])
+def test_noalloc_matches_transport_packets(prec, reemission):
     box = W.box_10pc()
     ncell, nsub = [32, 32, 32], [2, 2, 2]
     n, xn = _medium(32, 7)
@@ -35,6 +36,7 @@ def test_noalloc_matches_transport_packets(prec):
         emitter.set_sources([[0.3e16, -1.1e16, 0.7e16], [-2.0e16, 1.0e16, 0.0]],
                             [3e49, 1e49])
         ctx.set_fields(n, xn)
+        ctx.set_reemission(reemission, seed=4, iteration=1)
         # second call smaller than the first: the counters and the staged
         # records of the first call are still in the shared array
         for it, N in enumerate((50000, 12345)):
@@ -50,9 +52,11 @@ def test_noalloc_matches_transport_packets(prec):
                         assert np.array_equal(a[f], b[f]), (what, f)
                 else:
                     assert np.array_equal(a, b), what
-            for k in ("packets", "crossings", "hops", "absorbed", "escaped", "errors"):
+            for k in ("packets", "crossings", "hops", "absorbed", "escaped", "errors",
+                      "reemitted"):
                 assert ref[4][k] == got[4][k], k
             assert got[4]["packets"] == N
+            assert (got[4]["reemitted"] > 0) == (reemission > 0)
             assert got[4]["absorbed"] + got[4]["escaped"] == N
             assert got[4]["crossings"] == int(got[2].sum())
             Ja, Jb = ref[5], got[5]
