@@ -694,8 +694,22 @@ void orc_emit(const orc_grid *g, const double *src_pos, const double *src_L,
               int nsrc, uint64_t npackets, uint64_t first, uint64_t count,
               uint64_t seed, uint32_t iteration, double sigma, orc_photon *out,
               uint32_t *start_sub) {
+  orc_emit_problem(g, src_pos, src_L, nsrc, npackets, first, count, seed,
+                   iteration, sigma, 0, out, start_sub);
+}
+
+/* The emission of problem q of a problem set (no counterpart in the
+ * reference, which runs one grid per process): the source split keyed by
+ * seed ^ q, and the high counter word of packet k XOR q << 20 under the
+ * set's key; problem 0 is orc_emit. */
+void orc_emit_problem(const orc_grid *g, const double *src_pos,
+                      const double *src_L, int nsrc, uint64_t npackets,
+                      uint64_t first, uint64_t count, uint64_t seed,
+                      uint32_t iteration, double sigma, uint32_t problem,
+                      orc_photon *out, uint32_t *start_sub) {
   uint64_t *counts = (uint64_t *)malloc((size_t)nsrc * sizeof(uint64_t));
-  orc_source_counts(src_L, nsrc, npackets, seed, iteration, counts);
+  orc_source_counts(src_L, nsrc, npackets, seed ^ (uint64_t)problem, iteration,
+                    counts);
   const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   for (uint64_t j = 0; j < count; ++j) {
     const uint64_t k = first + j;
@@ -705,8 +719,9 @@ void orc_emit(const orc_grid *g, const double *src_pos, const double *src_L,
       ++src;
       acc += counts[src];
     }
-    const uint32_t c0[4] = {(uint32_t)k, (uint32_t)(k >> 32), iteration, 0u};
-    const uint32_t c1[4] = {(uint32_t)k, (uint32_t)(k >> 32), iteration, 1u};
+    const uint32_t hi = (uint32_t)(k >> 32) ^ (problem << 20);
+    const uint32_t c0[4] = {(uint32_t)k, hi, iteration, 0u};
+    const uint32_t c1[4] = {(uint32_t)k, hi, iteration, 1u};
     uint32_t r0[4], r1[4];
     orc_philox4x32_10(c0, key, r0);
     orc_philox4x32_10(c1, key, r1);

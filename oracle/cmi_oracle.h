@@ -145,6 +145,13 @@ void orc_emit(const orc_grid *g, const double *src_pos, const double *src_L,
               int nsrc, uint64_t npackets, uint64_t first, uint64_t count,
               uint64_t seed, uint32_t iteration, double sigma,
               orc_photon *out, uint32_t *start_sub);
+/* Problem q of a problem set: source split keyed by seed ^ q, counter high
+ * word XOR q << 20; problem 0 is orc_emit. */
+void orc_emit_problem(const orc_grid *g, const double *src_pos,
+                      const double *src_L, int nsrc, uint64_t npackets,
+                      uint64_t first, uint64_t count, uint64_t seed,
+                      uint32_t iteration, double sigma, uint32_t problem,
+                      orc_photon *out, uint32_t *start_sub);
 
 /* One plain iteration (emission + transport + ionization); returns the total
  * number of cell crossings and writes the mean |dx_n| to *l1. */

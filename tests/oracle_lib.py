@@ -85,6 +85,10 @@ def load_oracle():
     lib.orc_emit.argtypes = [C.POINTER(OrcGrid), _dp, _dp, C.c_int, C.c_uint64,
                              C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32,
                              C.c_double, C.c_void_p, _u32p]
+    lib.orc_emit_problem.argtypes = [C.POINTER(OrcGrid), _dp, _dp, C.c_int,
+                                     C.c_uint64, C.c_uint64, C.c_uint64,
+                                     C.c_uint64, C.c_uint32, C.c_double,
+                                     C.c_uint32, C.c_void_p, _u32p]
     lib.orc_iteration.argtypes = [C.POINTER(OrcGrid), _dp, _dp, C.c_int,
                                   C.c_uint64, C.c_uint64, C.c_uint32,
                                   C.c_double, C.c_double, C.c_int, C.c_int,
@@ -184,16 +188,23 @@ class OracleGrid:
                                               alpha, form)
 
     def emit(self, src_pos, src_L, npackets, seed, iteration, sigma=6.3e-22,
-             first=0, count=None):
+             first=0, count=None, problem=None):
+        """orc_emit, or orc_emit_problem for problem q of a problem set."""
         if count is None:
             count = npackets - first
         sp = np.ascontiguousarray(src_pos, dtype=np.float64).reshape(-1)
         sl = np.ascontiguousarray(src_L, dtype=np.float64)
         out = np.zeros(count, PHOTON_DTYPE)
         ss = np.zeros(count, np.uint32)
-        self.lib.orc_emit(C.byref(self.g), ptr(sp, _dp), ptr(sl, _dp), len(sl),
-                          npackets, first, count, seed, iteration, sigma,
-                          out.ctypes.data, ptr(ss, _u32p))
+        if problem is None:
+            self.lib.orc_emit(C.byref(self.g), ptr(sp, _dp), ptr(sl, _dp), len(sl),
+                              npackets, first, count, seed, iteration, sigma,
+                              out.ctypes.data, ptr(ss, _u32p))
+        else:
+            self.lib.orc_emit_problem(C.byref(self.g), ptr(sp, _dp), ptr(sl, _dp),
+                                      len(sl), npackets, first, count, seed,
+                                      iteration, sigma, int(problem),
+                                      out.ctypes.data, ptr(ss, _u32p))
         return out, ss
 
     def iteration(self, src_pos, src_L, npackets, seed, iteration,

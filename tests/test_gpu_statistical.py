@@ -7,16 +7,28 @@ FP64 update) against the reference's own transport and update
 DensitySubGrid::compute_neutral_fraction with its hard-coded alpha = 4e-19,
 built from the reference sources in oracle/_ref) on the host cores.
 
-The two legs use independent packet streams (GPU seeds 1-10, CPU seeds
-101-110).  Per radial bin (width 0.05 R_s) the seed-means of x_n must agree
-within 3 sigma of the seed-to-seed scatter, sigma^2 = var_gpu / 10 +
-var_cpu / 10, in at least 90 % of the bins and within 4.5 sigma in all
-(26 bins at 3 sigma would fail one bin in 14 runs by chance alone); both
-legs must find the analytic Stromgren radius within 2 %."""
+The two legs use independent packet streams (GPU seeds 1-16, CPU seeds
+101-116).  Per radial bin (width 0.05 R_s) the seed-means of x_n are
+compared through z = |mean_gpu - mean_cpu| / sigma, sigma^2 = var_gpu / 16
++ var_cpu / 16 (the seed-to-seed scatter).  "Agreement within 3 sigma" is a
+test at the 3-sigma false-alarm level (0.27 %, two-sided) for the field as a
+whole, not for each of its 26 bins separately (26 independent 3-sigma tests
+would fail one bin in 14 runs by chance alone):
+
+  - every bin: z below the Bonferroni threshold of that level over the m
+    bins, from Student's t with the bin's Welch-Satterthwaite degrees of
+    freedom (sigma is estimated from the seeds), about 4.5 here;
+  - all bins together: sum of z^2 (each scaled by (df - 2) / df, the
+    inverse variance of its t distribution) below the 0.27 % upper quantile
+    of chi-square with m degrees of freedom (a small offset common to many
+    bins fails this even when no single bin does).
+
+Both legs must find the analytic Stromgren radius within 2 %."""
 import os
 
 import numpy as np
 import pytest
+from scipy import stats
 
 import parallelcmacionize_amd as cmi
 from parallelcmacionize_amd import workloads as W
@@ -27,7 +39,8 @@ NC = 64
 NSUB = [4, 4, 4]
 N = 1_000_000
 ITER = 10
-SEEDS = 10
+SEEDS = 16
+LEVEL = 0.0027  # two-sided false-alarm rate of a 3-sigma test
 ALPHA = W.ALPHA_REF   # the reference update's own alpha
 
 
@@ -89,7 +102,8 @@ def test_converged_field_matches_reference_within_3_sigma(oracle):
     ref = O.load_ref(fast=False)
     if ref is None:
         pytest.skip("oracle/_ref not built (needs the reference checkout)")
-    nthreads = min(16, os.cpu_count() or 1)
+    # the host threads this process is given (OMP_NUM_THREADS), else all
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
     Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, ALPHA)
     idx, nb = radial_bins(Rs)
     g = np.array([bin_means(gpu_leg(s), idx, nb)[0]
@@ -103,13 +117,24 @@ def test_converged_field_matches_reference_within_3_sigma(oracle):
     # floor: bins where both legs are deterministic to rounding
     sig = np.maximum(sig, 1e-9 * np.maximum(np.abs(dg), np.abs(dc)) + 1e-15)
     z = np.abs(dg - dc)[use] / sig[use]
-    print("bins", int(use.sum()), "max z %.2f" % z.max(), "mean z %.2f" % z.mean(),
+    m = int(use.sum())
+    vg = (g.var(0, ddof=1) / SEEDS)[use]
+    vc = (c.var(0, ddof=1) / SEEDS)[use]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        df = (vg + vc) ** 2 / (vg ** 2 / (SEEDS - 1) + vc ** 2 / (SEEDS - 1))
+    df = np.where(np.isfinite(df), df, 2 * SEEDS - 2)
+    thr = stats.t.ppf(1 - LEVEL / (2 * m), df)
+    chi2_max = stats.chi2.ppf(1 - LEVEL, m)
+    # z^2 scaled to unit mean (a t variable's variance is df / (df - 2))
+    z2 = z ** 2 * (df - 2) / df
+    print("bins", m, "max z %.2f" % z.max(), "max z/threshold %.2f" % (z / thr).max(),
+          "sum z^2 %.1f (bound %.1f)" % (z2.sum(), chi2_max),
           "frac within 3 sigma %.3f" % np.mean(z <= 3))
     for k in np.nonzero(use)[0]:
         print("r/Rs %.2f  gpu %.4e  cpu %.4e  sigma %.2e" % (0.05 * k + 0.025,
               dg[k], dc[k], sig[k]))
-    assert np.mean(z <= 3) >= 0.9
-    assert z.max() <= 4.5
+    assert (z <= thr).all()
+    assert z2.sum() <= chi2_max
     box = W.box_10pc()
     Rg = np.mean([W.ionized_radius(gpu_leg(s), box, [NC] * 3) for s in (1, 2)])
     Rc = np.mean([W.ionized_radius(x, box, [NC] * 3) for x in xs[:2]])

@@ -625,3 +625,28 @@ def test_optical_depth_of_uniform_box_is_chord(oracle):
     assert (ex != 0).all()
     end = p["pos"] + chord[:, None] * p["dir"]
     assert np.allclose(q["pos"], end, rtol=0, atol=1e-9 * L)
+
+
+def test_problem_emission_is_the_set_emission(oracle):
+    """orc_emit_problem: problem 0 is orc_emit; problem q's packet k draws
+    the counters of packet number k + q * 2^52 (the high counter word XOR
+    q << 20), and its source split is keyed by seed ^ q."""
+    from parallelcmacionize_amd import workloads as W
+    box = W.box_10pc()
+    og = O.OracleGrid(oracle, box, [16] * 3, [2, 2, 2])
+    pos3 = [[0.1e16, 0.2e16, -0.3e16], [1e16, -2e16, 3e16], [-4e16, 0.0, 5e15]]
+    L3 = [1e49, 3e48, 7e48]
+    a, sa = og.emit(pos3, L3, 5001, 9, 2)
+    b, sb = og.emit(pos3, L3, 5001, 9, 2, problem=0)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    q, N = 5, 3000
+    one = og.emit([[0.0, 0.0, 0.0]], [1e49], N, 9, 2, problem=q)[0]
+    base = q << 52
+    ref = og.emit([[0.0, 0.0, 0.0]], [1e49], base + N, 9, 2, first=base, count=N)[0]
+    assert np.array_equal(one, ref)
+    p3, _ = og.emit(pos3, L3, N, 9, 2, problem=q)
+    counts = np.zeros(3, np.uint64)
+    Ls = np.array(L3)
+    oracle.orc_source_counts(O.ptr(Ls, O._dp), 3, N, 9 ^ q, 2, O.ptr(counts, O._u64p))
+    for s in range(3):
+        assert int((p3["pos"][:, 0] == pos3[s][0]).sum()) == int(counts[s])
