@@ -13,16 +13,22 @@
 //   DensitySubGridGrid   the grid of DensitySubGrids a driver builds
 //                        (testDensitySubGrid_omp.cpp:1805-1877); owns one
 //                        cmi_ctx
-//   DensitySubGrid       a view of one subgrid: interact (per packet and per
-//                        buffer), the cell setters/getters, neighbours
+//   DensitySubGrid       one subgrid: interact (per packet and per buffer),
+//                        the cell setters/getters, neighbours, active
+//                        buffers and the copy reduction; a view into a
+//                        DensitySubGridGrid, or stand-alone (the reference's
+//                        constructor and copy constructor) with a
+//                        one-subgrid grid of its own
 //
 // Errors: the reference aborts through cmac_error (Error.hpp:104-113); this
 // interface throws cmi::Error carrying the ABI's error code and message
 // instead, so a caller can recover.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -185,7 +191,21 @@ class DensitySubGridGrid;
 // One subgrid of a DensitySubGridGrid.
 class DensitySubGrid {
  public:
+  // subgrid `index` of grid g (the grid hands these out)
   DensitySubGrid(DensitySubGridGrid *g, unsigned int index) : g_(g), index_(index) {}
+  // DensitySubGrid.hpp:516: a stand-alone subgrid of ncell cells in box
+  // (anchor box[0..2], sides box[3..5]); n = 1e8, x_n = 1e-6, J = 0.  It
+  // owns a one-subgrid grid on the GPU.
+  inline DensitySubGrid(const double *box, const int *ncell,
+                        int precision = CMI_PREC_FP64);
+  // DensitySubGrid.hpp:618: a copy of a subgrid (the reference's subgrid
+  // copies): the original's geometry, n and x_n, J = 0, in a one-subgrid
+  // grid of its own.  Deposits into a copy stay in the copy until
+  // update_intensities adds them to the original.
+  inline DensitySubGrid(const DensitySubGrid &original);
+  inline DensitySubGrid(DensitySubGrid &&other) noexcept;
+  DensitySubGrid &operator=(const DensitySubGrid &) = delete;
+  inline ~DensitySubGrid();
   // DensitySubGrid::interact (DensitySubGrid.hpp:1180): one packet.
   inline int interact(Photon &photon, int input_direction);
   // The packet loop of do_photon_traversal (testDensitySubGrid_omp.cpp:
@@ -199,7 +219,22 @@ class DensitySubGrid {
   // DensitySubGrid::compute_optical_depth (DensitySubGrid.hpp:1482): walk to
   // the subgrid's edge, adding the path's optical depth to the packet's.
   inline int compute_optical_depth(Photon &photon, int input_direction);
+  // DensitySubGrid.hpp:820, 832: the neighbour table as the driver set it.
+  // A grid's transport routes by the table its geometry defines; shoot()
+  // checks that every neighbour a driver set agrees with it (CMI_ERR_ARG
+  // otherwise).  The per-subgrid walks do not read it: a driver that routes
+  // on the host reads its own values back.
   inline unsigned int get_neighbour(int direction) const;
+  inline void set_neighbour(int direction, unsigned int ngb);
+  // DensitySubGrid.hpp:878-892: the index of the subgrid's active buffer in
+  // each direction, for drivers that route buffers on the host
+  inline unsigned int get_active_buffer(int direction) const;
+  inline void set_active_buffer(int direction, unsigned int index);
+  // DensitySubGrid.hpp:1083-1089: x_n from the original, J reset (called on
+  // a copy after the original's ionization update)
+  inline void update_neutral_fractions(const DensitySubGrid &original);
+  // DensitySubGrid.hpp:1096-1108: add a copy's J to this subgrid's
+  inline void update_intensities(const DensitySubGrid &copy);
   // DensitySubGrid.hpp:483-506 (public in the reference for unit tests)
   inline int get_output_direction(const int three_index[3]) const;
   // DensitySubGrid.hpp:428-469
@@ -212,10 +247,12 @@ class DensitySubGrid {
   inline double get_intensity_integral(int ix, int iy, int iz) const;
   inline const double *get_neutral_fraction() const;
   unsigned int index() const { return index_; }
+  DensitySubGridGrid &grid() { return *g_; }
 
  private:
   DensitySubGridGrid *g_;
   unsigned int index_;
+  std::unique_ptr<DensitySubGridGrid> own_;  // stand-alone subgrids and copies
 };
 
 // The grid of subgrids (testDensitySubGrid_omp.cpp:1805-1877) on one GPU.
@@ -230,18 +267,26 @@ class DensitySubGridGrid {
     nsubgrid_ = ns;
     ncell_ = nc;
     cps_ = nc / ns;
+    for (int a = 0; a < 6; ++a) box_[a] = box[a];
     for (int a = 0; a < 3; ++a) {
       sc_[a] = ncell[a] / nsub[a];
       nsub_[a] = nsub[a];
       // DensitySubGrid.hpp:519-520 with the subgrid box of
       // testDensitySubGrid_omp.cpp:1806-1808
-      const double side = box[3 + a] / nsub[a];
-      cell_size_[a] = side / sc_[a];
-      inv_cell_size_[a] = sc_[a] / side;
+      side_[a] = box[3 + a] / nsub[a];
+      cell_size_[a] = side_[a] / sc_[a];
+      inv_cell_size_[a] = sc_[a] / side_[a];
     }
     n_.assign(nc, 1.e8);
     xn_.assign(nc, 1.e-6);
     J_.assign(nc, 0.);
+    ngb_geometry_.resize(27 * ns);
+    for (uint64_t s = 0; s < ns; ++s)
+      check(cmi_get_neighbours(ctx_, (uint32_t)s, &ngb_geometry_[27 * s]),
+            "cmi_get_neighbours");
+    ngb_declared_ = ngb_geometry_;
+    active_.assign(27 * ns, NEIGHBOUR_OUTSIDE);
+    subgrids_.reserve(ns);
     for (uint64_t s = 0; s < ns; ++s) subgrids_.emplace_back(this, (unsigned)s);
   }
   ~DensitySubGridGrid() { cmi_destroy(ctx_); }
@@ -262,6 +307,7 @@ class DensitySubGridGrid {
   // testDensitySubGrid_omp.cpp:2274-2401) with the GPU's own routing.
   cmi_stats shoot(uint64_t npackets, uint64_t seed, uint32_t iteration,
                   double sigma = 6.3e-22) {
+    check_neighbours();
     sync_in();
     cmi_stats st{};
     check(cmi_transport(ctx_, npackets, seed, iteration, sigma, &st), "cmi_transport");
@@ -325,18 +371,71 @@ class DensitySubGridGrid {
   void check(int rc, const char *what) {
     if (rc != CMI_OK) throw Error(rc, std::string(what) + ": " + cmi_last_error(ctx_));
   }
+  // The neighbours a driver set (set_neighbour) against the ones the
+  // transport routes by (the grid's geometry, testDensitySubGrid_omp.cpp:
+  // 1836-1874).
+  void check_neighbours() const {
+    if (!ngb_dirty_) return;
+    for (size_t k = 0; k < ngb_declared_.size(); ++k) {
+      if (ngb_declared_[k] != ngb_geometry_[k])
+        throw Error(CMI_ERR_ARG,
+                    "set_neighbour: subgrid " + std::to_string(k / 27) +
+                        ", direction " + std::to_string(k % 27) + ": " +
+                        std::to_string(ngb_declared_[k]) + " differs from the " +
+                        "grid's neighbour " + std::to_string(ngb_geometry_[k]));
+    }
+    ngb_dirty_ = false;
+  }
 
  private:
   friend class DensitySubGrid;
   cmi_ctx *ctx_ = nullptr;
   int precision_;
   size_t nsubgrid_ = 0, ncell_ = 0, cps_ = 0;
+  double box_[6];
   int sc_[3], nsub_[3];
-  double cell_size_[3], inv_cell_size_[3];
+  double side_[3], cell_size_[3], inv_cell_size_[3];
   std::vector<DensitySubGrid> subgrids_;
   std::vector<double> n_, xn_, J_;
+  std::vector<uint32_t> ngb_geometry_, ngb_declared_, active_;
+  mutable bool ngb_dirty_ = false;
   bool dirty_fields_ = true, dirty_J_ = false, host_stale_ = false;
 };
+
+inline DensitySubGrid::DensitySubGrid(const double *box, const int *ncell,
+                                      int precision)
+    : g_(nullptr), index_(0) {
+  const int one[3] = {1, 1, 1};
+  own_.reset(new DensitySubGridGrid(box, ncell, one, precision));
+  g_ = own_.get();
+}
+
+inline DensitySubGrid::DensitySubGrid(const DensitySubGrid &original)
+    : g_(nullptr), index_(0) {
+  DensitySubGridGrid &og = *original.g_;
+  og.sync_out();
+  // the original's subgrid box, as testDensitySubGrid_omp.cpp:1799-1808
+  // computes it (anchor = box anchor + index * side)
+  const size_t s = original.index_;
+  const int ix = (int)(s / ((size_t)og.nsub_[1] * og.nsub_[2]));
+  const int iy = (int)((s / og.nsub_[2]) % og.nsub_[1]);
+  const int iz = (int)(s % og.nsub_[2]);
+  const double sub[6] = {og.box_[0] + ix * og.side_[0], og.box_[1] + iy * og.side_[1],
+                         og.box_[2] + iz * og.side_[2], og.side_[0], og.side_[1],
+                         og.side_[2]};
+  const int one[3] = {1, 1, 1};
+  own_.reset(new DensitySubGridGrid(sub, og.sc_, one, og.precision_));
+  g_ = own_.get();
+  const size_t cps = og.cps_;
+  std::memcpy(g_->n_.data(), &og.n_[s * cps], cps * sizeof(double));
+  std::memcpy(g_->xn_.data(), &og.xn_[s * cps], cps * sizeof(double));
+  g_->dirty_fields_ = true;
+}
+
+inline DensitySubGrid::DensitySubGrid(DensitySubGrid &&other) noexcept
+    : g_(other.g_), index_(other.index_), own_(std::move(other.own_)) {}
+
+inline DensitySubGrid::~DensitySubGrid() = default;
 
 inline int DensitySubGrid::interact(Photon &photon, int input_direction) {
   g_->sync_out();
@@ -386,10 +485,50 @@ inline int DensitySubGrid::compute_optical_depth(Photon &photon, int input_direc
 }
 
 inline unsigned int DensitySubGrid::get_neighbour(int direction) const {
-  uint32_t n[27];
-  g_->check(cmi_get_neighbours(g_->ctx_, index_, n), "cmi_get_neighbours");
   if (direction < 0 || direction >= 27) throw Error(CMI_ERR_ARG, "output_direction");
-  return n[direction];
+  return g_->ngb_declared_[27 * (size_t)index_ + direction];
+}
+
+inline void DensitySubGrid::set_neighbour(int direction, unsigned int ngb) {
+  if (direction < 0 || direction >= 27) throw Error(CMI_ERR_ARG, "output_direction");
+  g_->ngb_declared_[27 * (size_t)index_ + direction] = ngb;
+  // a stand-alone subgrid or copy is routed by its driver: nothing to check
+  if (!own_) g_->ngb_dirty_ = true;
+}
+
+inline unsigned int DensitySubGrid::get_active_buffer(int direction) const {
+  if (direction < 0 || direction >= 27) throw Error(CMI_ERR_ARG, "direction");
+  return g_->active_[27 * (size_t)index_ + direction];
+}
+
+inline void DensitySubGrid::set_active_buffer(int direction, unsigned int index) {
+  if (direction < 0 || direction >= 27) throw Error(CMI_ERR_ARG, "direction");
+  g_->active_[27 * (size_t)index_ + direction] = index;
+}
+
+inline void DensitySubGrid::update_neutral_fractions(const DensitySubGrid &original) {
+  if (original.g_->cps_ != g_->cps_)
+    throw Error(CMI_ERR_ARG, "update_neutral_fractions: cell counts differ");
+  original.g_->sync_out();
+  g_->sync_out();
+  const size_t cps = g_->cps_;
+  std::memcpy(&g_->xn_[index_ * cps], &original.g_->xn_[original.index_ * cps],
+              cps * sizeof(double));
+  std::fill(g_->J_.begin() + index_ * cps, g_->J_.begin() + (index_ + 1) * cps, 0.);
+  g_->dirty_fields_ = true;
+  g_->dirty_J_ = true;
+}
+
+inline void DensitySubGrid::update_intensities(const DensitySubGrid &copy) {
+  if (copy.g_->cps_ != g_->cps_)
+    throw Error(CMI_ERR_ARG, "update_intensities: cell counts differ");
+  copy.g_->sync_out();
+  g_->sync_out();
+  const size_t cps = g_->cps_;
+  const double *src = &copy.g_->J_[copy.index_ * cps];
+  double *dst = &g_->J_[index_ * cps];
+  for (size_t i = 0; i < cps; ++i) dst[i] += src[i];
+  g_->dirty_J_ = true;
 }
 
 inline size_t DensitySubGrid::get_number_of_cells() const { return g_->cps_; }

@@ -1,8 +1,10 @@
 // Exercises the C++ host interface (parallelcmacionize_amd/host/cmi.hpp) the
 // way the reference's own programs use DensitySubGrid: the 54 known answers
 // of testDensitySubGridDirections.cpp:105-167, per-packet versus per-buffer
-// interact, neighbour tables, and a short Stromgren run through shoot() and
-// compute_neutral_fraction().  Exit code 0 = pass.
+// interact, the omp driver's neighbour set-up loop (testDensitySubGrid_omp.cpp:
+// 1836-1874) and active buffers, subgrid copies with their reduction
+// (2405-2410, 2429-2434), a stand-alone subgrid, and a short Stromgren run
+// through shoot() and compute_neutral_fraction().  Exit code 0 = pass.
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -162,6 +164,147 @@ int main() {
           "outside neighbour");
     CHECK(TravelDirections::output_to_input_direction(TRAVELDIRECTION_EDGE_Y_PN) ==
               TRAVELDIRECTION_EDGE_Y_NP, "output_to_input");
+
+    // ---- the omp driver's set-up loop, in its order ----
+    {
+      DensitySubGridGrid g2(box, ncell, nsub);
+      const int sc[3] = {ncell[0] / nsub[0], ncell[1] / nsub[1], ncell[2] / nsub[2]};
+      for (int ix = 0; ix < nsub[0]; ++ix)
+        for (int iy = 0; iy < nsub[1]; ++iy)
+          for (int iz = 0; iz < nsub[2]; ++iz) {
+            const unsigned index = (ix * nsub[1] + iy) * nsub[2] + iz;
+            DensitySubGrid &this_grid = g2[index];
+            for (int i = 0; i < TRAVELDIRECTION_NUMBER; ++i) {
+              this_grid.set_neighbour(i, NEIGHBOUR_OUTSIDE);
+              this_grid.set_active_buffer(i, NEIGHBOUR_OUTSIDE);
+            }
+            for (int nix = -1; nix < 2; ++nix)
+              for (int niy = -1; niy < 2; ++niy)
+                for (int niz = -1; niz < 2; ++niz) {
+                  const int cix = ix + nix, ciy = iy + niy, ciz = iz + niz;
+                  if (cix >= 0 && cix < nsub[0] && ciy >= 0 && ciy < nsub[1] &&
+                      ciz >= 0 && ciz < nsub[2]) {
+                    const int three_index[3] = {nix * sc[0], niy * sc[1], niz * sc[2]};
+                    const int ngbi = this_grid.get_output_direction(three_index);
+                    this_grid.set_neighbour(
+                        ngbi, (unsigned)((cix * nsub[1] + ciy) * nsub[2] + ciz));
+                  }
+                }
+          }
+      CHECK(g2[7].get_neighbour(TRAVELDIRECTION_CORNER_NNN) == 0 &&
+                g2[0].get_neighbour(TRAVELDIRECTION_FACE_X_N) == NEIGHBOUR_OUTSIDE &&
+                g2[0].get_neighbour(TRAVELDIRECTION_INSIDE) == 0,
+            "driver-built neighbours");
+      CHECK(g2[5].get_active_buffer(TRAVELDIRECTION_FACE_Z_P) == NEIGHBOUR_OUTSIDE,
+            "active buffer initialised by the driver");
+      g2[5].set_active_buffer(TRAVELDIRECTION_FACE_Z_P, 42);
+      CHECK(g2[5].get_active_buffer(TRAVELDIRECTION_FACE_Z_P) == 42, "active buffer");
+      g2.set_sources({0., 0., 0.}, {4.26e49});
+      const cmi_stats st = g2.shoot(20000, 1, 0);  // the driver's table agrees
+      CHECK(st.absorbed + st.escaped == 20000, "shoot after the set-up loop");
+      // a neighbour that disagrees with the grid is refused at the transport
+      g2[3].set_neighbour(TRAVELDIRECTION_FACE_X_P, 5);
+      bool refused = false;
+      try {
+        g2.shoot(1000, 1, 1);
+      } catch (const Error &e) {
+        refused = e.code() == CMI_ERR_ARG;
+      }
+      CHECK(refused, "wrong neighbour refused");
+      g2[3].set_neighbour(TRAVELDIRECTION_FACE_X_P, 7);
+      g2.shoot(1000, 1, 1);
+    }
+
+    // ---- subgrid copies and their reduction (omp driver 2405-2410, 2429-2434) ----
+    {
+      DensitySubGridGrid orig(box, ncell, nsub), whole(box, ncell, nsub);
+      for (size_t s = 0; s < orig.size(); ++s)
+        for (size_t i = 0; i < orig[s].get_number_of_cells(); ++i) {
+          orig[s].set_neutral_fraction((unsigned)i, 1e-3 * (1 + (i % 7)));
+          whole[s].set_neutral_fraction((unsigned)i, 1e-3 * (1 + (i % 7)));
+        }
+      DensitySubGrid copy(orig[7]);  // DensitySubGrid.hpp:618
+      CHECK(copy.get_number_of_cells() == orig[7].get_number_of_cells(), "copy size");
+      PhotonBuffer first, second, all;
+      for (PhotonBuffer *b : {&first, &second, &all}) {
+        b->set_subgrid_index(7);
+        b->set_direction(TRAVELDIRECTION_INSIDE);
+      }
+      for (unsigned int k = 0; k < PHOTONBUFFER_SIZE; ++k) {
+        PhotonBuffer &half = k < 100 ? first : second;
+        half[half.get_next_free_photon()] = single[k];
+        all[all.get_next_free_photon()] = single[k];
+      }
+      for (unsigned int k = 0; k < PHOTONBUFFER_SIZE; ++k) {
+        Photon &h = k < 100 ? first[k] : second[k - 100];
+        for (Photon *q : {&all[k], &h}) {
+          q->set_position(0.1 * pc, 0.2 * pc, 0.3 * pc);
+          q->set_target_optical_depth(0.05 + 0.01 * (k % 10));
+        }
+      }
+      std::vector<int> e1(100), e2(100), ea(200);
+      orig[7].interact(first, e1.data());
+      copy.interact(second, e2.data());  // deposits stay in the copy
+      whole[7].interact(all, ea.data());
+      for (unsigned int k = 100; k < PHOTONBUFFER_SIZE; ++k) {
+        CHECK(e2[k - 100] == ea[k], "copy walks like the original, packet %u", k);
+        for (int a = 0; a < 3; ++a)
+          CHECK(second[k - 100].get_position()[a] == all[k].get_position()[a],
+                "copy position, packet %u", k);
+      }
+      orig[7].update_intensities(copy);
+      const int n16 = ncell[0] / nsub[0];
+      double maxrel = 0., total = 0.;
+      for (int ix = 0; ix < n16; ++ix)
+        for (int iy = 0; iy < n16; ++iy)
+          for (int iz = 0; iz < n16; ++iz) {
+            const double a = orig[7].get_intensity_integral(ix, iy, iz);
+            const double b = whole[7].get_intensity_integral(ix, iy, iz);
+            total += b;
+            if (b > 0.) maxrel = std::fmax(maxrel, std::fabs(a - b) / b);
+          }
+      CHECK(total > 0. && maxrel < 1e-12, "reduced J of the copy: max rel %.3e", maxrel);
+      // form 1 (cancellation-free): the two J differ by summation order only,
+      // which the reference form's cancellation would amplify by about aa
+      orig.compute_neutral_fraction(4.26e49, 200, 4e-19, 1);
+      whole.compute_neutral_fraction(4.26e49, 200, 4e-19, 1);
+      copy.update_neutral_fractions(orig[7]);  // DensitySubGrid.hpp:1083
+      const double *xo = orig[7].get_neutral_fraction();
+      const double *xc = copy.get_neutral_fraction();
+      const double *xw = whole[7].get_neutral_fraction();
+      bool same = true, close = true;
+      for (size_t i = 0; i < copy.get_number_of_cells(); ++i) {
+        same = same && xc[i] == xo[i];
+        close = close && std::fabs(xo[i] - xw[i]) <= 1e-12 * std::fabs(xw[i]);
+      }
+      CHECK(same, "copy takes the original's neutral fractions");
+      CHECK(close, "copies reduce to the single-grid result");
+      CHECK(copy.get_intensity_integral(3, 4, 5) == 0., "copy J reset");
+    }
+
+    // ---- a stand-alone subgrid (DensitySubGrid.hpp:516) walks like the same
+    //      subgrid of a grid ----
+    {
+      const double side = 10 * pc / 2;
+      const double sb[6] = {box[0] + side, box[1] + side, box[2] + side, side, side, side};
+      const int snc[3] = {16, 16, 16};
+      DensitySubGrid solo(sb, snc);
+      for (size_t i = 0; i < solo.get_number_of_cells(); ++i)
+        solo.set_neutral_fraction((unsigned)i, 1e-3 * (1 + (i % 7)));
+      bool ok = true;
+      for (unsigned int k = 0; k < 50; ++k) {
+        Photon a = single[k], b = single[k];
+        a.set_position(0.1 * pc, 0.2 * pc, 0.3 * pc);
+        b.set_position(0.1 * pc, 0.2 * pc, 0.3 * pc);
+        a.set_target_optical_depth(0.05 + 0.01 * (k % 10));
+        b.set_target_optical_depth(0.05 + 0.01 * (k % 10));
+        const int ea = solo.interact(a, TRAVELDIRECTION_INSIDE);
+        const int eb = grid[7].interact(b, TRAVELDIRECTION_INSIDE);
+        ok = ok && ea == eb && a.get_target_optical_depth() == b.get_target_optical_depth();
+        for (int d = 0; d < 3; ++d) ok = ok && a.get_position()[d] == b.get_position()[d];
+      }
+      CHECK(ok, "stand-alone subgrid walks bit for bit like the grid's subgrid 7");
+    }
 
     // ---- MemorySpace spill ----
     MemorySpace ms(8);
