@@ -81,9 +81,11 @@ int default_tile(int v, int cap) {
 int set_tile(cmi_ctx *c, const int tile[3]) {
   for (int a = 0; a < 3; ++a) {
     const int s = ilog2_exact(tile[a]);
-    if (s < 0 || tile[a] > c->sc[a] || c->sc[a] % tile[a] != 0)
+    // at most 64 cells per axis: the tiles engine counts a flight's steps
+    // along each axis in a 7-bit field
+    if (s < 0 || tile[a] > c->sc[a] || c->sc[a] % tile[a] != 0 || tile[a] > 64)
       return fail(c, CMI_ERR_ARG,
-                  "tile sizes must be powers of two dividing the subgrid");
+                  "tile sizes must be powers of two up to 64 dividing the subgrid");
   }
   for (int a = 0; a < 3; ++a) {
     c->tile[a] = tile[a];

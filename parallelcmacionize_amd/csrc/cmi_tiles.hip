@@ -600,6 +600,39 @@ struct AccAtomic<double> {
   __device__ static void add(double *p, float v) { atomicAdd(p, (double)v); }
 };
 
+// Per-axis step counters of a straight flight inside a tile, packed in one
+// register: axis 0 in bits 16-23, axis 1 in bits 8-15, axis 2 in bits 0-7.
+// The field of axis a starts at 64 - T_a plus the cells already behind the
+// packet along its direction (l_a for a positive direction, T_a - 1 - l_a
+// for a negative one) and grows by one per step along a, so the flight
+// leaves the tile exactly when a field reaches 64 (bit 6 of the field, the
+// guard).  One add and one mask test per crossing replace three index
+// updates and three bound tests, and the count of steps taken is the byte
+// sum of the register's growth.
+constexpr uint32_t kStepGuard = 0x404040u;
+constexpr uint32_t kStepInc0 = 1u << 16, kStepInc1 = 1u << 8, kStepInc2 = 1u;
+
+__device__ __forceinline__ uint32_t step_field(int l, float d, int T) {
+  return (uint32_t)(64 - T + (d > 0.f ? l : T - 1 - l));
+}
+__device__ __forceinline__ uint32_t pack_steps(int l0, int l1, int l2, float d0,
+                                               float d1, float d2, int T0,
+                                               int T1, int T2) {
+  return (step_field(l0, d0, T0) << 16) | (step_field(l1, d1, T1) << 8) |
+         step_field(l2, d2, T2);
+}
+// Cell index along an axis from its step field (T_a or -1 once the flight
+// has left the tile along that axis).
+__device__ __forceinline__ int field_index(uint32_t f, float d, int T) {
+  const int dist = (int)f - (64 - T);
+  return d > 0.f ? dist : T - 1 - dist;
+}
+// Steps taken between two step registers of one straight flight (the sum of
+// the byte-wise growth; no field ever decreases).
+__device__ __forceinline__ uint32_t steps_between(uint32_t lp, uint32_t lp0) {
+  return __builtin_amdgcn_sad_u8(lp - lp0, 0u, 0u);
+}
+
 // kReemit: an absorbed packet is reemitted with probability G.reemit_p
 // (do_reemission, testDensitySubGrid_omp.cpp:675-702): a new isotropic
 // direction and target optical depth from the absorption point, in the same
@@ -629,7 +662,6 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   const uint32_t ndesc = *ndesc_ptr;
   const int T0 = G.T[0], T1 = G.T[1], T2 = G.T[2];
   const int st0 = T1 * T2, st1 = T2;
-  const int max_steps = T0 + T1 + T2 + 4;
   // per-lane counters of this launch (32 bits: a lane walks at most a few
   // hundred thousand crossings per launch); widened for the final sums
   uint32_t my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0, my_err = 0;
@@ -694,248 +726,245 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     __syncthreads();
 
     // per-lane packet state
-    bool live = false, pending = false;
-    int outcome = 0;  // 0 absorbed, 1 moved to a neighbour tile, 2 escaped, 3 error
-    int l0 = 0, l1 = 0, l2 = 0, off = 0, steps = 0, exit_axis = 0;
+    bool live = false, pending = false, bad = false;
+    int off = 0;
     int so0 = 0, so1 = 0, so2 = 0;  // signed LDS strides along the direction
-    int sg0 = 0, sg1 = 0, sg2 = 0;  // index steps along the direction
+    uint32_t lp = 0, lp0 = 0;       // step fields now and at the leg's start
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
     float tm0 = 0.f, tm1 = 0.f, tm2 = 0.f, td0 = 0.f, td1 = 0.f, td2 = 0.f;
     float t = 0.f, tau = 0.f;
-    uint32_t my_pid = 0, vis_cross = 0, n_reemit = 0;
-
+    uint32_t my_pid = 0, vis_acc = 0, n_reemit = 0;
     float kap_c = 0.f;  // opacity of the lane's current cell (read ahead)
+
     for (;;) {
-      // refill when refill_batch lanes have finished or wait for a record,
-      // or when no lane is live; one ballot while every lane is walking
-      const uint64_t idle = __ballot(!live);
-      if (idle != 0ull &&
-          (idle == ~0ull ||
-           __popcll(__ballot(!live && (pending || has_nxt))) >= refill_batch)) {
-        // ---- write out finished packets ----
-        if (pending) {
-          my_cross += vis_cross;
-          my_abs += (outcome == 0);
-          my_esc += (outcome == 2);
-          my_err += (outcome == 3);
-          if (kList) {
-            po.crossings[my_pid] += vis_cross;
-            po.hops[my_pid] += 1;
-            if (outcome != 1) po.status[my_pid] = outcome == 0 ? 0 : (outcome == 2 ? 1 : -2);
-          }
+      // ---- write out finished packets ----
+      bool moving = false;
+      int wl0 = 0, wl1 = 0, wl2 = 0, exit_axis = 0;
+      if (pending) {
+        const bool exited = !bad && (lp & kStepGuard) != 0u;
+        if (!bad && !exited) {
+          // absorbed: the partial last step, up to the target optical depth
+          // (reemitting kernels deposited it at the absorption and left
+          // tau = 0)
+          const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
+          AccAtomic<Acc>::add(&accL[off], sa);
+          t = t + sa;
         }
-        const bool moving = pending && outcome == 1;
-        const uint64_t mv = __ballot(moving);
-        uint32_t sbase = 0;
-        if (mv) {
-          const int leader = __ffsll((long long)mv) - 1;
-          if (lane == leader) sbase = atomicAdd(&s_surv, (uint32_t)__popcll(mv));
-          sbase = __shfl(sbase, leader, 64);
+        const uint32_t vis = bad ? 0u : vis_acc + steps_between(lp, lp0) + (exited ? 0u : 1u);
+        wl0 = field_index((lp >> 16) & 0x7fu, d0, T0);
+        wl1 = field_index((lp >> 8) & 0x7fu, d1, T1);
+        wl2 = field_index(lp & 0x7fu, d2, T2);
+        exit_axis = (lp & (64u << 16)) ? 0 : ((lp & (64u << 8)) ? 1 : 2);
+        const int gx = ox + wl0, gy = oy + wl1, gz = oz + wl2;
+        const bool in_box = (unsigned)gx < (unsigned)PN0 && (unsigned)gy < (unsigned)PN1 &&
+                            (unsigned)gz < (unsigned)PN2;
+        // 0 absorbed, 1 moved to a neighbour tile, 2 escaped, 3 error
+        const int outcome = bad ? 3 : (!exited ? 0 : (in_box ? 1 : 2));
+        moving = outcome == 1;
+        my_cross += vis;
+        my_abs += (outcome == 0);
+        my_esc += (outcome == 2);
+        my_err += (outcome == 3);
+        if (kList) {
+          po.crossings[my_pid] += vis;
+          po.hops[my_pid] += 1;
+          if (outcome != 1) po.status[my_pid] = outcome == 0 ? 0 : (outcome == 2 ? 1 : -2);
         }
-        if (pending) {
-          // position at the exit, in the new tile's frame
-          const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
-          const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
-          if (moving) {
-            // the new tile differs along the exit axis only
-            Rec32 r;
-            r.a.x = (exit_axis == 0) ? (d0 > 0.f ? 0.f : (float)T0) : q0;
-            r.a.y = (exit_axis == 1) ? (d1 > 0.f ? 0.f : (float)T1) : q1;
-            r.a.z = (exit_axis == 2) ? (d2 > 0.f ? 0.f : (float)T2) : q2;
-            r.a.w = d0;
-            r.b.x = d1;
-            r.b.y = d2;
-            r.b.z = tau;
-            r.b.w = __uint_as_float(pack_cell(gx, gy, gz));
-            const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
-            Pout[slot] = r;
-            const float de = exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2);
-            const bool up = de > 0.f;
-            dest_out[slot] = (uint32_t)(exit_axis == 0 ? (up ? nb_xp : nb_xn)
-                                        : exit_axis == 1 ? (up ? nb_yp : nb_yn)
-                                                         : (up ? nb_zp : nb_zn));
-            if (kList) pid_out[slot] = my_pid;
-            // the rank among this visit's exits through this face; the
-            // face's base is added once the visit has ended
-            const uint32_t f = 2u * (uint32_t)exit_axis + (up ? 0u : 1u);
-            rel_out[slot] = (f << 29) | atomicAdd(&s_face[f], 1u);
-          } else if (kList) {
-            po.final_a[my_pid] = make_float4(q0, q1, q2, d0);
-            po.final_b[my_pid] = make_float4(d1, d2, tau, 0.f);
-            po.final_tile[my_pid] = tile;
-          }
-          pending = false;
-        }
-        // ---- take the prefetched record, prefetch the next one ----
-        const bool take = !live && has_nxt;
-        if (take) {
-          p0 = na.x;
-          p1 = na.y;
-          p2 = na.z;
-          d0 = na.w;
-          d1 = nb.x;
-          d2 = nb.y;
-          tau = nb.z;
-          my_pid = npid;
-          int gx, gy, gz;
-          unpack_cell(__float_as_uint(nb.w), gx, gy, gz);
-          l0 = gx - ox;
-          l1 = gy - oy;
-          l2 = gz - oz;
-          ++my_hops;
-          vis_cross = 0;
-          n_reemit = 0;
-          steps = 0;
-          t = 0.f;
-          has_nxt = false;
-          if ((unsigned)l0 >= (unsigned)T0 || (unsigned)l1 >= (unsigned)T1 ||
-              (unsigned)l2 >= (unsigned)T2) {
-            pending = true;  // a record that does not belong to this tile
-            outcome = 3;
-          } else {
-            const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
-                        i2 = __builtin_amdgcn_rcpf(d2);
-            tm0 = d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
-                           : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX);
-            tm1 = d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
-                           : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX);
-            tm2 = d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
-                           : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX);
-            td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
-            td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
-            td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
-            so0 = d0 > 0.f ? st0 : -st0;
-            so1 = d1 > 0.f ? st1 : -st1;
-            so2 = d2 > 0.f ? 1 : -1;
-            sg0 = d0 > 0.f ? 1 : -1;
-            sg1 = d1 > 0.f ? 1 : -1;
-            sg2 = so2;
-            off = (l0 * T1 + l1) * T2 + l2;
-            kap_c = kapL[off];
-            live = true;
-          }
-        }
-        const uint64_t tm = __ballot(take);
-        if (tm) {
-          const int leader = __ffsll((long long)tm) - 1;
-          uint32_t kb = 0;
-          if (lane == leader) kb = atomicAdd(&s_next, (uint32_t)__popcll(tm));
-          kb = __shfl(kb, leader, 64);
-          if (take) {
-            const uint32_t k = kb + (uint32_t)__popcll(tm & lt);
-            if (k < D.count) {
-              const uint32_t src = idx ? idx[D.start + k] : D.start + k;
-              na = Pin[src].a;
-              nb = Pin[src].b;
-              if (kList) npid = pid_in[src];
-              has_nxt = true;
-            }
-          }
-        }
-        if (__ballot(live || pending || has_nxt) == 0) break;
       }
-      if (!live) continue;
-      // ---- one crossing ----
-      const float tn = fminf(tm0, fminf(tm1, tm2));
-      const float seg = fmaxf(tn - t, 0.f);
-      const float kap = kap_c;
-      const float dtau = seg * kap;
-      ++vis_cross;
-      const bool absorbed = dtau >= tau;
-      // the path deposited in this cell: to its face, or the partial last
-      // step up to the target optical depth
-      const float sa =
-          absorbed ? (kap > 0.f ? tau * __builtin_amdgcn_rcpf(kap) : 0.f) : seg;
-      AccAtomic<Acc>::add(&accL[off], sa);
-      if (absorbed) {
-        t = t + sa;
-        if (kReemit) {
-          const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
-          const u32x4 r = philox4x32_10(
-              u32x4{__float_as_uint(q0), __float_as_uint(q1), __float_as_uint(q2),
-                    (uint32_t)tile ^ (n_reemit << 24)},
-              G.reemit_key[0], G.reemit_key[1]);
-          const float k24 = 1.f / 16777216.f;
-          if (((float)(r.x >> 8) + 0.5f) * k24 < G.reemit_p) {
-            // get_random_direction, testDensitySubGrid_omp.cpp:529-545, and
-            // tau = -ln u: FP32 draws on 24-bit uniforms in (0, 1)
-            const float cost = 2.f * (((float)(r.y >> 8) + 0.5f) * k24) - 1.f;
-            const float phi = 6.28318530717958647692f * (((float)(r.z >> 8) + 0.5f) * k24);
-            const float sint = sqrtf(fmaxf(1.f - cost * cost, 0.f));
-            float sp, cp;
-            sincosf(phi, &sp, &cp);
-            d0 = sint * cp * (float)P.unit_scale[0];
-            d1 = sint * sp * (float)P.unit_scale[1];
-            d2 = cost * (float)P.unit_scale[2];
-            tau = -__logf(((float)(r.w >> 8) + 0.5f) * k24);
-            p0 = q0;
-            p1 = q1;
-            p2 = q2;
-            t = 0.f;
-            ++n_reemit;
-            ++my_re;
-            steps = 0;
-            const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
-                        i2 = __builtin_amdgcn_rcpf(d2);
-            tm0 = d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
-                           : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX);
-            tm1 = d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
-                           : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX);
-            tm2 = d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
-                           : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX);
-            tm0 = fmaxf(tm0, 0.f);
-            tm1 = fmaxf(tm1, 0.f);
-            tm2 = fmaxf(tm2, 0.f);
-            td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
-            td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
-            td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
-            so0 = d0 > 0.f ? st0 : -st0;
-            so1 = d1 > 0.f ? st1 : -st1;
-            so2 = d2 > 0.f ? 1 : -1;
-            sg0 = d0 > 0.f ? 1 : -1;
-            sg1 = d1 > 0.f ? 1 : -1;
-            sg2 = so2;
-            continue;
-          }
-        }
-        live = false;
-        pending = true;
-        outcome = 0;
-        continue;
+      const uint64_t mv = __ballot(moving);
+      uint32_t sbase = 0;
+      if (mv) {
+        const int leader = __ffsll((long long)mv) - 1;
+        if (lane == leader) sbase = atomicAdd(&s_surv, (uint32_t)__popcll(mv));
+        sbase = __shfl(sbase, leader, 64);
       }
-      tau -= dtau;
-      t = tn > t ? tn : t;  // fmaxf without the NaN canonicalisation
-      const bool a0 = tm0 == tn;
-      const bool a1 = !a0 && (tm1 == tn);
-      const bool a2 = !a0 && !a1;
-      l0 += a0 ? sg0 : 0;
-      l1 += a1 ? sg1 : 0;
-      l2 += a2 ? sg2 : 0;
-      off += a0 ? so0 : (a1 ? so1 : so2);
-      tm0 += a0 ? td0 : 0.f;
-      tm1 += a1 ? td1 : 0.f;
-      tm2 += a2 ? td2 : 0.f;
-      const bool out0 = (unsigned)l0 >= (unsigned)T0, out1 = (unsigned)l1 >= (unsigned)T1,
-                 out2 = (unsigned)l2 >= (unsigned)T2;
-      const bool out = out0 | out1 | out2;
-      // walk bound, for reemitted flights only: every other step moves one
-      // cell along one axis, always in the direction's sign along that axis,
-      // so a flight leaves the tile within T0 + T1 + T2 steps
-      if (kReemit) ++steps;
-      const int nsteps = kReemit ? steps : (int)vis_cross;
-      kap_c = kapL[out ? 0 : off];  // the next cell, read ahead (unused if out)
-      if (out | (kReemit && nsteps > max_steps)) {
-        if (out) {
-          exit_axis = out0 ? 0 : (out1 ? 1 : 2);
-          const int gx = ox + l0, gy = oy + l1, gz = oz + l2;
-          const bool in_box = (unsigned)gx < (unsigned)PN0 && (unsigned)gy < (unsigned)PN1 &&
-                              (unsigned)gz < (unsigned)PN2;
-          outcome = in_box ? 1 : 2;
+      if (pending) {
+        // position at the exit (or the absorption), in the tile frame
+        const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
+        if (moving) {
+          // the new tile differs along the exit axis only
+          Rec32 r;
+          r.a.x = (exit_axis == 0) ? (d0 > 0.f ? 0.f : (float)T0) : q0;
+          r.a.y = (exit_axis == 1) ? (d1 > 0.f ? 0.f : (float)T1) : q1;
+          r.a.z = (exit_axis == 2) ? (d2 > 0.f ? 0.f : (float)T2) : q2;
+          r.a.w = d0;
+          r.b.x = d1;
+          r.b.y = d2;
+          r.b.z = tau;
+          r.b.w = __uint_as_float(pack_cell(ox + wl0, oy + wl1, oz + wl2));
+          const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
+          Pout[slot] = r;
+          const float de = exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2);
+          const bool up = de > 0.f;
+          dest_out[slot] = (uint32_t)(exit_axis == 0 ? (up ? nb_xp : nb_xn)
+                                      : exit_axis == 1 ? (up ? nb_yp : nb_yn)
+                                                       : (up ? nb_zp : nb_zn));
+          if (kList) pid_out[slot] = my_pid;
+          // the rank among this visit's exits through this face; the
+          // face's base is added once the visit has ended
+          const uint32_t f = 2u * (uint32_t)exit_axis + (up ? 0u : 1u);
+          rel_out[slot] = (f << 29) | atomicAdd(&s_face[f], 1u);
+        } else if (kList) {
+          po.final_a[my_pid] = make_float4(q0, q1, q2, d0);
+          po.final_b[my_pid] = make_float4(d1, d2, tau, 0.f);
+          po.final_tile[my_pid] = tile;
+        }
+        pending = false;
+        bad = false;
+      }
+      // ---- take the prefetched record, prefetch the next one ----
+      const bool take = !live && has_nxt;
+      if (take) {
+        p0 = na.x;
+        p1 = na.y;
+        p2 = na.z;
+        d0 = na.w;
+        d1 = nb.x;
+        d2 = nb.y;
+        tau = nb.z;
+        my_pid = npid;
+        int gx, gy, gz;
+        unpack_cell(__float_as_uint(nb.w), gx, gy, gz);
+        const int l0 = gx - ox, l1 = gy - oy, l2 = gz - oz;
+        ++my_hops;
+        vis_acc = 0;
+        n_reemit = 0;
+        t = 0.f;
+        has_nxt = false;
+        if ((unsigned)l0 >= (unsigned)T0 || (unsigned)l1 >= (unsigned)T1 ||
+            (unsigned)l2 >= (unsigned)T2) {
+          pending = true;  // a record that does not belong to this tile
+          bad = true;
         } else {
-          outcome = 3;  // walk bound
+          const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
+                      i2 = __builtin_amdgcn_rcpf(d2);
+          // the next face along each axis; a position a rounding step past
+          // its cell's face starts at 0, so that t only ever grows
+          tm0 = fmaxf(d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
+                               : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX), 0.f);
+          tm1 = fmaxf(d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
+                               : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX), 0.f);
+          tm2 = fmaxf(d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
+                               : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX), 0.f);
+          td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
+          td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
+          td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+          so0 = d0 > 0.f ? st0 : -st0;
+          so1 = d1 > 0.f ? st1 : -st1;
+          so2 = d2 > 0.f ? 1 : -1;
+          lp = pack_steps(l0, l1, l2, d0, d1, d2, T0, T1, T2);
+          lp0 = lp;
+          off = (l0 * T1 + l1) * T2 + l2;
+          kap_c = kapL[off];
+          live = true;
         }
-        live = false;
-        pending = true;
+      }
+      const uint64_t tm = __ballot(take);
+      if (tm) {
+        const int leader = __ffsll((long long)tm) - 1;
+        uint32_t kb = 0;
+        if (lane == leader) kb = atomicAdd(&s_next, (uint32_t)__popcll(tm));
+        kb = __shfl(kb, leader, 64);
+        if (take) {
+          const uint32_t k = kb + (uint32_t)__popcll(tm & lt);
+          if (k < D.count) {
+            const uint32_t src = idx ? idx[D.start + k] : D.start + k;
+            na = Pin[src].a;
+            nb = Pin[src].b;
+            if (kList) npid = pid_in[src];
+            has_nxt = true;
+          }
+        }
+      }
+      const uint64_t live_mask = __ballot(live);
+      if ((live_mask | __ballot(pending || has_nxt)) == 0ull) break;
+      // ---- walk: until refill_batch of the wave's lanes have finished
+      //      while records are left for them, else until every lane has ----
+      const int nlive0 = __popcll(live_mask);
+      const int stop_at = __ballot(has_nxt) ? max(nlive0 - refill_batch, 0) : 0;
+      int nlive = nlive0;
+      while (nlive > stop_at) {
+        bool fin = false;
+        if (live) {
+          const float tn = fminf(tm0, fminf(tm1, tm2));
+          const float seg = tn - t;
+          const float dtau = seg * kap_c;
+          if (dtau >= tau) {
+            // absorbed in this cell
+            bool again = false;
+            if (kReemit) {
+              const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
+              AccAtomic<Acc>::add(&accL[off], sa);
+              t = t + sa;
+              tau = 0.f;
+              const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
+              const u32x4 r = philox4x32_10(
+                  u32x4{__float_as_uint(q0), __float_as_uint(q1), __float_as_uint(q2),
+                        (uint32_t)tile ^ (n_reemit << 24)},
+                  G.reemit_key[0], G.reemit_key[1]);
+              const float k24 = 1.f / 16777216.f;
+              if (((float)(r.x >> 8) + 0.5f) * k24 < G.reemit_p) {
+                // get_random_direction, testDensitySubGrid_omp.cpp:529-545,
+                // and tau = -ln u: FP32 draws on 24-bit uniforms in (0, 1)
+                const int l0 = field_index((lp >> 16) & 0x7fu, d0, T0);
+                const int l1 = field_index((lp >> 8) & 0x7fu, d1, T1);
+                const int l2 = field_index(lp & 0x7fu, d2, T2);
+                vis_acc += steps_between(lp, lp0) + 1u;
+                const float cost = 2.f * (((float)(r.y >> 8) + 0.5f) * k24) - 1.f;
+                const float phi = 6.28318530717958647692f * (((float)(r.z >> 8) + 0.5f) * k24);
+                const float sint = sqrtf(fmaxf(1.f - cost * cost, 0.f));
+                float sp, cp;
+                sincosf(phi, &sp, &cp);
+                d0 = sint * cp * (float)P.unit_scale[0];
+                d1 = sint * sp * (float)P.unit_scale[1];
+                d2 = cost * (float)P.unit_scale[2];
+                tau = -__logf(((float)(r.w >> 8) + 0.5f) * k24);
+                p0 = q0;
+                p1 = q1;
+                p2 = q2;
+                t = 0.f;
+                ++n_reemit;
+                ++my_re;
+                const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
+                            i2 = __builtin_amdgcn_rcpf(d2);
+                tm0 = fmaxf(d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
+                                     : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX), 0.f);
+                tm1 = fmaxf(d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
+                                     : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX), 0.f);
+                tm2 = fmaxf(d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
+                                     : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX), 0.f);
+                td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
+                td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
+                td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+                so0 = d0 > 0.f ? st0 : -st0;
+                so1 = d1 > 0.f ? st1 : -st1;
+                so2 = d2 > 0.f ? 1 : -1;
+                lp = pack_steps(l0, l1, l2, d0, d1, d2, T0, T1, T2);
+                lp0 = lp;
+                again = true;
+              }
+            }
+            fin = !again;
+          } else {
+            AccAtomic<Acc>::add(&accL[off], seg);
+            tau -= dtau;
+            t = tn;
+            const bool a0 = tm0 == tn;
+            const bool a1 = !a0 && (tm1 == tn);
+            off += a0 ? so0 : (a1 ? so1 : so2);
+            lp += a0 ? kStepInc0 : (a1 ? kStepInc1 : kStepInc2);
+            tm0 += a0 ? td0 : 0.f;
+            tm1 += a1 ? td1 : 0.f;
+            tm2 += (!a0 && !a1) ? td2 : 0.f;
+            fin = (lp & kStepGuard) != 0u;
+            kap_c = kapL[fin ? 0 : off];  // the next cell, read ahead
+          }
+          if (fin) {
+            live = false;
+            pending = true;
+          }
+        }
+        nlive -= __popcll(__ballot(fin));
       }
     }
     __syncthreads();
