@@ -24,6 +24,39 @@ sys.path.insert(0, ROOT)
 METRIC = ("cell-crossings/s at 256³ clumpy; wall time to converged x_n "
           "(plain vs warm)")
 PEAK_HBM = 8.0e12  # B/s, MI355X HBM3E
+# wave64 VALU instruction issue of the whole chip: 256 CUs x 4 SIMDs, a wave64
+# vector instruction every 4 cycles per SIMD, 2.4 GHz peak engine clock
+PEAK_VALU = 256 * 4 * 2.4e9 / 4
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def host_cpu():
+    """Model and core counts of the host (from /proc/cpuinfo)."""
+    model, cores, logical = "unknown", set(), 0
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "processor":
+                logical += 1
+            elif k == "model name":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+                cores.add((phys, core))
+    except OSError:
+        pass
+    return {"model": model, "physical_cores": len(cores) or None,
+            "logical_cpus": logical or os.cpu_count()}
+
+
+def host_threads():
+    """The host threads this process is given: OMP_NUM_THREADS where the
+    environment sets it (16 per GPU on the MI355X hosts), else every CPU."""
+    return int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
 
 
 def log(*a):
@@ -90,11 +123,59 @@ def cpu_baseline(box, ncell, nsub, n, xn, pos, L, seed, iteration,
     pk, ss, cr = sample(S)
     t = run(pk, ss)
     c64.close()
+    cpu = host_cpu()
     return {"value": cr / t, "unit": "crossings/s", "cores": nthreads,
             "kind": kind,
+            "cpu_model": cpu["model"], "host_physical_cores": cpu["physical_cores"],
+            "host_logical_cpus": cpu["logical_cpus"],
+            "driver": ("the reference's interact, PhotonBuffer and "
+                       "MemorySpace::add_photons under a task loop with a "
+                       "private grid copy per thread (no subgrid locks or "
+                       "work stealing)"),
             "sample": ("%d packets of the same workload (same field state, "
                        "same sources, iteration %d), %d crossings in %.2f s "
                        "on %d threads" % (S, iteration, cr, t, nthreads))}
+
+
+def error_vs_fp64(box, ncell, nsub, n, xn, pos, L, seed, iteration, ctx32):
+    """One iteration from the same field state through the all-FP64 GPU walk
+    (the reference's arithmetic) and through the FP32 tiles context ctx32 on
+    the same packets: the per-cell intensity integrals and the updated x_n
+    of the FP32 run against the FP64 ones."""
+    import parallelcmacionize_amd as cmi
+    from parallelcmacionize_amd import workloads as W
+    N = 100_000_000
+    out = {}
+    with cmi.Context(box, [ncell] * 3, nsub, cmi.PREC_FP64) as c64:
+        c64.set_fields(n, xn)
+        c64.set_sources(pos, L)
+        s64 = c64.transport(N, seed, iteration)
+        J64, _ = c64.get_fields()
+        c64.update_ionization(c64.total_luminosity(), N, W.ALPHA_B, form=1)
+        _, x64 = c64.get_fields()
+    ctx32.set_fields(xn=xn)
+    ctx32.reset_intensity()
+    s32 = ctx32.transport(N, seed, iteration)
+    J32, _ = ctx32.get_fields()
+    ctx32.update_ionization(ctx32.total_luminosity(), N, W.ALPHA_B, form=1)
+    _, x32 = ctx32.get_fields()
+    big = J64 > 1e-3 * J64.max()
+    rel = np.abs(J32[big] - J64[big]) / J64[big]
+    d = np.abs(x32 - x64)
+    out = {
+        "reference": ("all-FP64 GPU walk on the same packets (%.0e, seed %d, "
+                      "iteration %d) from the same field state" % (N, seed, iteration)),
+        "crossings_fp64": s64["crossings"], "crossings_fp32": s32["crossings"],
+        "absorbed_fp64": s64["absorbed"], "absorbed_fp32": s32["absorbed"],
+        "J_total_rel": float(abs(J32.sum() / J64.sum() - 1.0)),
+        "J_cell_rel_median": float(np.median(rel)),
+        "J_cell_rel_p99": float(np.percentile(rel, 99)),
+        "xn_abs_mean": float(d.mean()), "xn_abs_max": float(d.max()),
+        "fp64_walk_seconds": s64["seconds"],
+    }
+    log("error vs FP64: J total %.2e, x_n mean %.2e max %.2e"
+        % (out["J_total_rel"], out["xn_abs_mean"], out["xn_abs_max"]))
+    return out
 
 
 def copy_calibration(nbytes):
@@ -168,6 +249,12 @@ def warm_start_section(args, box, ncell, nsub, n_ref, pos, L):
         "ionized_radius_fixed_point_m": r_ref,
         "ionized_radius_plain_rel": r_plain / r_ref - 1.0,
         "ionized_radius_warm_rel": r_warm / r_ref - 1.0,
+        "accuracy_reference": ("the fixed point of the same packet set (mean "
+                               "|dx_n| < 1e-5); fixed points of two packet seeds "
+                               "differ by 7.4e-3 mean |dx_n| (the field's Monte "
+                               "Carlo noise), so the errors compare the two "
+                               "starts on one packet set, not against the "
+                               "physical solution"),
         "network": os.path.relpath(args.net, ROOT),
     }
 
@@ -207,6 +294,11 @@ def precision_section(args):
         log("precision %s: %.3e crossings/s" % (name, cross / secs))
     Rs = W.stromgren_radius(W.Q_STROMGREN, W.N_H, W.ALPHA_B)
     d = np.abs(xs["fp32_tiles"] - xs["fp64_walk"])
+    out["fp32_crossings_note"] = (
+        "the FP32 engine steps tied axes one at a time: packets leaving the "
+        "source, which sits on a cell corner, make zero-length steps that the "
+        "FP64 walk merges, so its crossing count is %.2f %% higher"
+        % (100.0 * (out["fp32_tiles"]["crossings"] / out["fp64_walk"]["crossings"] - 1.0)))
     out["xn_error_fp32_vs_fp64"] = {"mean_abs": float(d.mean()),
                                     "max_abs": float(d.max()),
                                     "mean_abs_ionized": float(d[xs["fp64_walk"] < 0.5].mean())}
@@ -259,6 +351,8 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-error", action="store_true",
+                    help="skip the FP32-vs-FP64 error of the headline workload")
     ap.add_argument("--precision-packets", type=float, default=1e8)
     ap.add_argument("--precision-iterations", type=int, default=10)
     ap.add_argument("--batched-iterations", type=int, default=5)
@@ -380,22 +474,51 @@ def main():
             "avg_kernel_s": avg_kernel,
         },
     }
-    traffic_file = os.path.join(ROOT, "profiles",
-                                "tiles_process_kernel_256clumpy_traffic.json")
-    if (args.engine == "tiles" and ncell == 256 and N == 100_000_000 and
-            prec == cmi.PREC_FP32 and os.path.exists(traffic_file)):
-        with open(traffic_file) as f:
-            tr = json.load(f)
-    else:
-        tr = None
-    if tr and tr.get("steps") == args.steps and tr.get("warmup") == args.warmup:
-        # HBM bytes per launch of the same kernel on the same workload, from
-        # the calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-        out["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
-        out["roofline"]["traffic_source"] = tr["source"]
+    # counter records of this kernel on this workload (the profiled command
+    # is the bench's own main measurement; the records are per launch, so
+    # they hold for any step count)
+    headline = (args.engine == "tiles" and ncell == 256 and N == 100_000_000 and
+                prec == cmi.PREC_FP32)
+    tr = sq = None
+    if headline:
+        for name in ("tiles_process_kernel_256clumpy_traffic.json",
+                     "tiles_process_kernel_256clumpy_sq.json"):
+            path = os.path.join(PROFILES, name)
+            if os.path.exists(path):
+                with open(path) as f:
+                    rec = json.load(f)
+                if "traffic" in name:
+                    tr = rec
+                else:
+                    sq = rec
+    R = out["roofline"]
+    if tr:
+        # HBM bytes per launch from the calibrated FETCH_SIZE / WRITE_SIZE runs
+        R["traffic"] = tr["hbm_bytes_per_launch"]
+        R["traffic_frac"] = tr["hbm_bytes_per_launch"] / avg_kernel / PEAK_HBM
+        R["traffic_source"] = tr["source"]
+    if sq:
+        # vector-instruction issue from the SQ counters: the rate this
+        # kernel's launches issue wave64 VALU instructions against the chip's
+        # issue rate, and the lanes busy per instruction
+        v = sq["valu_wave_instr_per_launch"]
+        R["issue"] = {"achieved": v / avg_kernel, "peak": PEAK_VALU,
+                      "unit": "wave64 VALU instructions/s",
+                      "frac": v / avg_kernel / PEAK_VALU,
+                      "valu_wave_instr_per_launch": v,
+                      "active_lanes_per_valu": sq["active_lanes_per_valu"],
+                      "valu_wave_instr_per_crossing": sq["valu_wave_instr_per_crossing"],
+                      "source": sq["source"]}
+    if tr and sq:
+        # what binds: the larger of the measured HBM and VALU-issue fractions
+        R["limiter"] = ("valu_issue" if R["issue"]["frac"] > R["traffic_frac"]
+                        else "hbm")
+    xn_now = ctx.get_fields()[1]
+    if headline and not args.no_error:
+        out["error_vs_fp64"] = error_vs_fp64(box, ncell, nsub, n, xn_now, pos, L,
+                                             seed, args.warmup + args.steps, ctx)
     if not args.no_cpu:
-        xn_now = ctx.get_fields()[1]
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads = host_threads()
         log("cpu baseline on %d threads" % nthreads)
         out["cpu_baseline"] = cpu_baseline(
             box, ncell, nsub, n, xn_now, pos, L, seed,
