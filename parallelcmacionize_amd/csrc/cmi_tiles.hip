@@ -196,15 +196,26 @@ __device__ __forceinline__ int set_emit(const SetSources &S, uint64_t kg,
   return p;
 }
 
+// Where a source's packets start, the same for all of them: the tile of the
+// start cell, the source position in that tile's frame (cells), the packed
+// global cell, and the source's offset within the tile's round-0 segment.
+struct SrcTile {
+  float px, py, pz;
+  uint32_t tile;
+  uint32_t cell;
+  uint32_t off;
+  uint32_t pad[2];
+};
+
 // Seed and iteration come from S (device memory), so that a captured graph
 // of the iteration can be replayed with new values.  Round 0 is planned
 // before emission (the per-tile counts follow from the source split), so
 // every packet also goes straight into round 0's index list: its source's
-// share of the tile's segment starts at src_off[source].
+// share of the tile's segment starts at src[source].off.
 __global__ void tiles_emit_kernel(TileGeo G, const SetSources *__restrict__ S,
                                   uint64_t npackets, Rec32 *__restrict__ out,
                                   const uint32_t *__restrict__ seg,
-                                  const uint32_t *__restrict__ src_off,
+                                  const SrcTile *__restrict__ src_tile,
                                   uint32_t *__restrict__ idx) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < npackets) {
@@ -214,57 +225,56 @@ __global__ void tiles_emit_kernel(TileGeo G, const SetSources *__restrict__ S,
     const int p = set_emit(*S, k, S->iteration, (uint32_t)S->seed,
                            (uint32_t)(S->seed >> 32), pos, dir, &tau, &src, &rank);
     const ProbGeo &P = G.probs[p];
-    int g[3];
-    double pc[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      pc[a] = (pos[a] - P.box[a]) * P.inv_cell_size[a];
-      int c = (int)pc[a];
-      g[a] = c < 0 ? 0 : (c >= P.N[a] ? P.N[a] - 1 : c);
-    }
-    const int tx = g[0] >> G.tshift[0], ty = g[1] >> G.tshift[1],
-              tz = g[2] >> G.tshift[2];
-    const uint32_t tile = (uint32_t)tile_index(P, tx, ty, tz);
+    const SrcTile st = src_tile[src];
     Rec32 r;
-    r.a.x = (float)(pc[0] - (double)(tx << G.tshift[0]));
-    r.a.y = (float)(pc[1] - (double)(ty << G.tshift[1]));
-    r.a.z = (float)(pc[2] - (double)(tz << G.tshift[2]));
+    r.a.x = st.px;
+    r.a.y = st.py;
+    r.a.z = st.pz;
     r.a.w = (float)(dir[0] * P.unit_scale[0]);
     r.b.x = (float)(dir[1] * P.unit_scale[1]);
     r.b.y = (float)(dir[2] * P.unit_scale[2]);
     r.b.z = (float)tau;
-    r.b.w = __uint_as_float(pack_cell(g[0], g[1], g[2]));
+    r.b.w = __uint_as_float(st.cell);
     out[k] = r;
     // in range whenever the source split sums to npackets (as it does by
     // construction); the check keeps a broken split from writing past idx
-    const uint64_t at = (uint64_t)seg[tile] + src_off[src] + rank;
+    const uint64_t at = (uint64_t)seg[st.tile] + st.off + rank;
     if (at < npackets) idx[at] = (uint32_t)k;
   }
 }
 
 // Tile of every source's start cell (the cell the emission kernel files its
 // packets under) gets that source's packet count; the old count is the
-// source's offset within the tile's round-0 segment.
+// source's offset within the tile's round-0 segment.  The start cell and
+// the source's position in its tile's frame are kept for the emission.
 __global__ void tiles_source_hist_kernel(TileGeo G,
                                          const SetSources *__restrict__ S,
                                          uint32_t *__restrict__ hist,
-                                         uint32_t *__restrict__ src_off) {
+                                         SrcTile *__restrict__ src_tile) {
   for (int p = 0; p < S->nprob; ++p) {
     const ProbGeo &P = G.probs[p];
     const int b = S->src_begin[p], e = S->src_begin[p + 1];
     for (int s = b + (int)threadIdx.x; s < e; s += blockDim.x) {
       const uint64_t c = S->src_cum[s] - (s > b ? S->src_cum[s - 1] : 0);
-      if (c == 0) continue;
       int g[3];
+      double pc[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const double pc = (S->pos[s][a] - P.box[a]) * P.inv_cell_size[a];
-        const int cc = (int)pc;
+        pc[a] = (S->pos[s][a] - P.box[a]) * P.inv_cell_size[a];
+        const int cc = (int)pc[a];
         g[a] = cc < 0 ? 0 : (cc >= P.N[a] ? P.N[a] - 1 : cc);
       }
-      const int tile = tile_index(P, g[0] >> G.tshift[0], g[1] >> G.tshift[1],
-                                  g[2] >> G.tshift[2]);
-      src_off[s] = atomicAdd(&hist[tile], (uint32_t)c);
+      const int tx = g[0] >> G.tshift[0], ty = g[1] >> G.tshift[1],
+                tz = g[2] >> G.tshift[2];
+      SrcTile st;
+      st.tile = (uint32_t)tile_index(P, tx, ty, tz);
+      st.px = (float)(pc[0] - (double)(tx << G.tshift[0]));
+      st.py = (float)(pc[1] - (double)(ty << G.tshift[1]));
+      st.pz = (float)(pc[2] - (double)(tz << G.tshift[2]));
+      st.cell = pack_cell(g[0], g[1], g[2]);
+      st.off = c ? atomicAdd(&hist[st.tile], (uint32_t)c) : 0u;
+      st.pad[0] = st.pad[1] = 0u;
+      src_tile[s] = st;
     }
   }
 }
@@ -1136,7 +1146,7 @@ struct TilesEngine {
   uint32_t *total = nullptr;
   unsigned *ticket = nullptr;
   uint32_t *h_total = nullptr;  // pinned, [3]
-  uint32_t *src_off = nullptr;  // each source's offset in its round-0 tile segment
+  SrcTile *src_tile = nullptr;  // each source's start tile, cell and round-0 offset
   int tile_key[3] = {0, 0, 0};
   std::vector<hipEvent_t> ev;   // start/stop of every process launch
   // problem set
@@ -1173,7 +1183,7 @@ static void free_engine(TilesEngine *e) {
   hipFree(e->ndesc);
   hipFree(e->total);
   hipFree(e->ticket);
-  hipFree(e->src_off);
+  hipFree(e->src_tile);
   if (e->h_total) hipHostFree(e->h_total);
   hipFree(e->d_probs);
   hipFree(e->d_tile_prob);
@@ -1283,7 +1293,7 @@ static int ensure_engine(cmi_ctx *c, const std::vector<ProbGeo> &probs,
   if (err == hipSuccess) err = hipMalloc(&e->total, 3 * sizeof(uint32_t));
   if (err == hipSuccess) err = hipMalloc(&e->ticket, sizeof(unsigned));
   if (err == hipSuccess) err = hipHostMalloc(&e->h_total, 3 * sizeof(uint32_t));
-  if (err == hipSuccess) err = hipMalloc(&e->src_off, kMaxSetSources * sizeof(uint32_t));
+  if (err == hipSuccess) err = hipMalloc(&e->src_tile, kMaxSetSources * sizeof(SrcTile));
   if (err == hipSuccess) err = hipMalloc(&e->d_probs, np * sizeof(ProbGeo));
   if (err == hipSuccess) err = hipMalloc(&e->d_tile_prob, ntiles * sizeof(uint16_t));
   if (err == hipSuccess) err = hipMalloc(&e->d_src, sizeof(SetSources));
@@ -1526,11 +1536,11 @@ int tiles_transport(cmi_ctx *c, uint64_t npackets, uint64_t seed,
   // round 0: per-tile counts from the source split, planned, then the
   // emission fills the records and the index list
   hipLaunchKernelGGL(tiles_source_hist_kernel, dim3(1), dim3(256), 0, c->stream, G,
-                     e->d_src, e->hist, e->src_off);
+                     e->d_src, e->hist, e->src_tile);
   launch_plan(c, G, R, false);
   const unsigned eb = (unsigned)((npackets + 255) / 256);
   hipLaunchKernelGGL(tiles_emit_kernel, dim3(eb), dim3(256), 0, c->stream, G, e->d_src,
-                     npackets, e->P[0], e->seg, e->src_off, e->idx);
+                     npackets, e->P[0], e->seg, e->src_tile, e->idx);
   CMI_TRY_HIP(c, hipGetLastError());
   PacketOut po{};
   uint32_t rounds = 0;
@@ -1594,7 +1604,11 @@ int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub
     return fail(c, CMI_ERR_HIP, "tiles list transport: copy failed");
   }
   hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, c->stream, c->d_cnt);
-  CMI_TRY_HIP(c, hipEventRecord(c->ev0, c->stream));
+  err = hipEventRecord(c->ev0, c->stream);
+  if (err != hipSuccess) {
+    cleanup();
+    return fail(c, CMI_ERR_HIP, "tiles list transport: event record failed");
+  }
   hipLaunchKernelGGL(tiles_load_list_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
                      0, c->stream, G, c->g, dp, ds, n, e->P[0], e->dest[0], e->pid[0],
                      e->hist);
@@ -1616,8 +1630,8 @@ int tiles_transport_packets(cmi_ctx *c, cmi_photon *p, const uint32_t *start_sub
   }
   hipLaunchKernelGGL(tiles_finish_list_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
                      0, c->stream, G, c->g, n, fa, dtile, fb, dp);
-  CMI_TRY_HIP(c, hipEventRecord(c->ev1, c->stream));
-  err = hipMemcpyAsync(p, dp, n * sizeof(PhotonRec), hipMemcpyDeviceToHost, c->stream);
+  err = hipEventRecord(c->ev1, c->stream);
+  if (err == hipSuccess) err = hipMemcpyAsync(p, dp, n * sizeof(PhotonRec), hipMemcpyDeviceToHost, c->stream);
   if (err == hipSuccess && status)
     err = hipMemcpyAsync(status, dst, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
   if (err == hipSuccess && crossings)

@@ -156,25 +156,30 @@ def test_tiles_results_independent_of_chunking():
     """How a round's packets are cut into workgroup visits (the chunk, which
     the plan also shrinks for rounds with few packets) changes only the order
     of the FP64 sums: every packet walks the same cells, so the crossings,
-    fates, tile visits and rounds agree exactly and J to FP64 rounding."""
+    fates, tile visits and rounds agree exactly and J to FP64 rounding.
+    2e6 packets: the first rounds hold more than a quarter of the resident
+    workgroups' worth of full chunks (192 x 8192 on 256 CUs), so the 8192 and
+    16384 chunks are compared uncut there and cut in the later rounds."""
     ncell, nsub = [64, 64, 64], [4, 4, 4]
     box = W.box_10pc()
     n, xn = _field(8, ncell, nsub, -6, -3)
     pos, L = W.random_sources(6, box, seed=12)
     out = []
-    for chunk in (256, 8192):
+    N = 2_000_000
+    for chunk in (256, 8192, 16384):
         with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
             ctx.set_engine(cmi.ENGINE_TILES, chunk_min=chunk)
             ctx.set_fields(n, xn)
             ctx.set_sources(pos, L)
-            s = ctx.transport(300000, 5, 1)
+            s = ctx.transport(N, 5, 1)
             J, _ = ctx.get_fields()
-            assert s["errors"] == 0 and s["absorbed"] + s["escaped"] == 300000
+            assert s["errors"] == 0 and s["absorbed"] + s["escaped"] == N
             out.append((s, J))
-    (sa, Ja), (sb, Jb) = out
-    for k in ("crossings", "absorbed", "escaped", "rounds", "hops"):
-        assert sa[k] == sb[k], k
-    assert np.allclose(Ja, Jb, rtol=1e-9, atol=0)
+    sa, Ja = out[0]
+    for sb, Jb in out[1:]:
+        for k in ("crossings", "absorbed", "escaped", "rounds", "hops"):
+            assert sa[k] == sb[k], k
+        assert np.allclose(Ja, Jb, rtol=1e-9, atol=0)
 
 
 @pytest.mark.parametrize("prec", [cmi.PREC_FP32, cmi.PREC_FP64])
