@@ -643,6 +643,20 @@ __device__ __forceinline__ uint32_t steps_between(uint32_t lp, uint32_t lp0) {
   return __builtin_amdgcn_sad_u8(lp - lp0, 0u, 0u);
 }
 
+// DDA set-up along one axis for a packet at p (cells, tile frame) in cell l
+// moving with direction component d: the path parameter of the next face
+// ahead (never negative: a position a rounding step past its cell's face
+// starts at 0, so that t only ever grows) and the path between faces; an
+// axis the packet does not move along never steps.  Computed for both signs
+// and selected, without branches.
+__device__ __forceinline__ void dda_axis(int l, float p, float d, float &tm,
+                                         float &td) {
+  const float i = __builtin_amdgcn_rcpf(d);
+  const float face = (float)(d > 0.f ? l + 1 : l);
+  tm = d != 0.f ? fmaxf((face - p) * i, 0.f) : FLT_MAX;
+  td = d != 0.f ? fabsf(i) : FLT_MAX;
+}
+
 // kReemit: an absorbed packet is reemitted with probability G.reemit_p
 // (do_reemission, testDensitySubGrid_omp.cpp:675-702): a new isotropic
 // direction and target optical depth from the absorption point, in the same
@@ -666,6 +680,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   __shared__ uint32_t s_next, s_surv, s_desc;
   __shared__ uint32_t s_face[6];
   __shared__ uint32_t s_fbase[6];
+  __shared__ uint32_t s_nbr[6];  // the tile's face neighbours, by exit face
 
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -721,6 +736,12 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     const int nb_zp = tile_index(P, ttx, tty, ttz + 1), nb_zn = tile_index(P, ttx, tty, ttz - 1);
     const int ox = ttx << G.tshift[0], oy = tty << G.tshift[1],
               oz = ttz << G.tshift[2];
+    // exit face f = 2 axis + (negative direction): its neighbour tile
+    if (threadIdx.x < 6) {
+      const int f = threadIdx.x;
+      s_nbr[f] = (uint32_t)(f == 0 ? nb_xp : f == 1 ? nb_xn : f == 2 ? nb_yp
+                            : f == 3 ? nb_yn : f == 4 ? nb_zp : nb_zn);
+    }
 
     // prefetched next record of this lane
     bool has_nxt = false;
@@ -736,7 +757,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     __syncthreads();
 
     // per-lane packet state
-    bool live = false, pending = false, bad = false;
+    bool live = false, pending = false;
     int off = 0;
     int so0 = 0, so1 = 0, so2 = 0;  // signed LDS strides along the direction
     uint32_t lp = 0, lp0 = 0;       // step fields now and at the leg's start
@@ -751,8 +772,8 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       bool moving = false;
       int wl0 = 0, wl1 = 0, wl2 = 0, exit_axis = 0;
       if (pending) {
-        const bool exited = !bad && (lp & kStepGuard) != 0u;
-        if (!bad && !exited) {
+        const bool exited = (lp & kStepGuard) != 0u;
+        if (!exited) {
           // absorbed: the partial last step, up to the target optical depth
           // (reemitting kernels deposited it at the absorption and left
           // tau = 0)
@@ -760,7 +781,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           AccAtomic<Acc>::add(&accL[off], sa);
           t = t + sa;
         }
-        const uint32_t vis = bad ? 0u : vis_acc + steps_between(lp, lp0) + (exited ? 0u : 1u);
+        const uint32_t vis = vis_acc + steps_between(lp, lp0) + (exited ? 0u : 1u);
         wl0 = field_index((lp >> 16) & 0x7fu, d0, T0);
         wl1 = field_index((lp >> 8) & 0x7fu, d1, T1);
         wl2 = field_index(lp & 0x7fu, d2, T2);
@@ -768,17 +789,16 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         const int gx = ox + wl0, gy = oy + wl1, gz = oz + wl2;
         const bool in_box = (unsigned)gx < (unsigned)PN0 && (unsigned)gy < (unsigned)PN1 &&
                             (unsigned)gz < (unsigned)PN2;
-        // 0 absorbed, 1 moved to a neighbour tile, 2 escaped, 3 error
-        const int outcome = bad ? 3 : (!exited ? 0 : (in_box ? 1 : 2));
+        // 0 absorbed, 1 moved to a neighbour tile, 2 escaped
+        const int outcome = !exited ? 0 : (in_box ? 1 : 2);
         moving = outcome == 1;
         my_cross += vis;
         my_abs += (outcome == 0);
         my_esc += (outcome == 2);
-        my_err += (outcome == 3);
         if (kList) {
           po.crossings[my_pid] += vis;
           po.hops[my_pid] += 1;
-          if (outcome != 1) po.status[my_pid] = outcome == 0 ? 0 : (outcome == 2 ? 1 : -2);
+          if (outcome != 1) po.status[my_pid] = outcome == 0 ? 0 : 1;
         }
       }
       const uint64_t mv = __ballot(moving);
@@ -805,14 +825,11 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           const uint32_t slot = D.start + sbase + (uint32_t)__popcll(mv & lt);
           Pout[slot] = r;
           const float de = exit_axis == 0 ? d0 : (exit_axis == 1 ? d1 : d2);
-          const bool up = de > 0.f;
-          dest_out[slot] = (uint32_t)(exit_axis == 0 ? (up ? nb_xp : nb_xn)
-                                      : exit_axis == 1 ? (up ? nb_yp : nb_yn)
-                                                       : (up ? nb_zp : nb_zn));
+          const uint32_t f = 2u * (uint32_t)exit_axis + (de > 0.f ? 0u : 1u);
+          dest_out[slot] = s_nbr[f];
           if (kList) pid_out[slot] = my_pid;
           // the rank among this visit's exits through this face; the
           // face's base is added once the visit has ended
-          const uint32_t f = 2u * (uint32_t)exit_axis + (up ? 0u : 1u);
           rel_out[slot] = (f << 29) | atomicAdd(&s_face[f], 1u);
         } else if (kList) {
           po.final_a[my_pid] = make_float4(q0, q1, q2, d0);
@@ -820,7 +837,6 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           po.final_tile[my_pid] = tile;
         }
         pending = false;
-        bad = false;
       }
       // ---- take the prefetched record, prefetch the next one ----
       const bool take = !live && has_nxt;
@@ -843,22 +859,20 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         has_nxt = false;
         if ((unsigned)l0 >= (unsigned)T0 || (unsigned)l1 >= (unsigned)T1 ||
             (unsigned)l2 >= (unsigned)T2) {
-          pending = true;  // a record that does not belong to this tile
-          bad = true;
+          // a record that does not belong to this tile: an error, written
+          // out at once (it never walks)
+          ++my_err;
+          if (kList) {
+            po.hops[my_pid] += 1;
+            po.status[my_pid] = -2;
+            po.final_a[my_pid] = make_float4(p0, p1, p2, d0);
+            po.final_b[my_pid] = make_float4(d1, d2, tau, 0.f);
+            po.final_tile[my_pid] = tile;
+          }
         } else {
-          const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
-                      i2 = __builtin_amdgcn_rcpf(d2);
-          // the next face along each axis; a position a rounding step past
-          // its cell's face starts at 0, so that t only ever grows
-          tm0 = fmaxf(d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
-                               : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX), 0.f);
-          tm1 = fmaxf(d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
-                               : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX), 0.f);
-          tm2 = fmaxf(d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
-                               : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX), 0.f);
-          td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
-          td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
-          td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+          dda_axis(l0, p0, d0, tm0, td0);
+          dda_axis(l1, p1, d1, tm1, td1);
+          dda_axis(l2, p2, d2, tm2, td2);
           so0 = d0 > 0.f ? st0 : -st0;
           so1 = d1 > 0.f ? st1 : -st1;
           so2 = d2 > 0.f ? 1 : -1;
@@ -886,12 +900,14 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           }
         }
       }
+      // (no lane is pending here: every one was written out above)
       const uint64_t live_mask = __ballot(live);
-      if ((live_mask | __ballot(pending || has_nxt)) == 0ull) break;
+      const uint64_t nxt_mask = __ballot(has_nxt);
+      if ((live_mask | nxt_mask) == 0ull) break;
       // ---- walk: until refill_batch of the wave's lanes have finished
       //      while records are left for them, else until every lane has ----
       const int nlive0 = __popcll(live_mask);
-      const int stop_at = __ballot(has_nxt) ? max(nlive0 - refill_batch, 0) : 0;
+      const int stop_at = nxt_mask ? max(nlive0 - refill_batch, 0) : 0;
       int nlive = nlive0;
       while (nlive > stop_at) {
         bool fin = false;
@@ -935,17 +951,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
                 t = 0.f;
                 ++n_reemit;
                 ++my_re;
-                const float i0 = __builtin_amdgcn_rcpf(d0), i1 = __builtin_amdgcn_rcpf(d1),
-                            i2 = __builtin_amdgcn_rcpf(d2);
-                tm0 = fmaxf(d0 > 0.f ? ((float)(l0 + 1) - p0) * i0
-                                     : (d0 < 0.f ? ((float)l0 - p0) * i0 : FLT_MAX), 0.f);
-                tm1 = fmaxf(d1 > 0.f ? ((float)(l1 + 1) - p1) * i1
-                                     : (d1 < 0.f ? ((float)l1 - p1) * i1 : FLT_MAX), 0.f);
-                tm2 = fmaxf(d2 > 0.f ? ((float)(l2 + 1) - p2) * i2
-                                     : (d2 < 0.f ? ((float)l2 - p2) * i2 : FLT_MAX), 0.f);
-                td0 = d0 != 0.f ? fabsf(i0) : FLT_MAX;
-                td1 = d1 != 0.f ? fabsf(i1) : FLT_MAX;
-                td2 = d2 != 0.f ? fabsf(i2) : FLT_MAX;
+                dda_axis(l0, p0, d0, tm0, td0);
+                dda_axis(l1, p1, d1, tm1, td1);
+                dda_axis(l2, p2, d2, tm2, td2);
                 so0 = d0 > 0.f ? st0 : -st0;
                 so1 = d1 > 0.f ? st1 : -st1;
                 so2 = d2 > 0.f ? 1 : -1;
