@@ -77,6 +77,10 @@ typedef struct {
   uint32_t reserved;
   double seconds;     /* device time of the transport (HIP events) */
   uint64_t reemitted; /* reemission events (an absorbed packet sent on) */
+  uint64_t walkout_crossings; /* tiles: the crossings (of those above) walked
+                                 in a walk-out launch */
+  uint64_t walkout_hops;      /* tiles: the tile visits (of hops) walked in a
+                                 walk-out launch */
 } cmi_stats;
 
 /* Tuning of the engines (0 = default for any field). */
@@ -94,6 +98,13 @@ typedef struct {
   int walk_blocks;   /* walk: persistent grid size */
   int lds_fp32;      /* tiles: 1 = FP32 LDS accumulators for J (flushed into
                         FP64), 0 = FP64 */
+  int finish_packets;  /* tiles: when the packets left (counted at the start
+                          and every 8 rounds) are at most this many, one
+                          walk-out launch walks them all to the end
+                          (opacities read from memory, FP64 atomic
+                          deposits; the same cells, crossings and fates as
+                          the rounds).  0 = the default threshold, < 0 =
+                          never.  Single contexts without reemission only. */
 } cmi_tuning;
 
 typedef struct cmi_ctx cmi_ctx;

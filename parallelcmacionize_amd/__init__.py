@@ -49,7 +49,8 @@ class Stats(C.Structure):
                 ("hops", C.c_uint64), ("absorbed", C.c_uint64),
                 ("escaped", C.c_uint64), ("errors", C.c_uint64),
                 ("rounds", C.c_uint32), ("reserved", C.c_uint32),
-                ("seconds", C.c_double), ("reemitted", C.c_uint64)]
+                ("seconds", C.c_double), ("reemitted", C.c_uint64),
+                ("walkout_crossings", C.c_uint64), ("walkout_hops", C.c_uint64)]
 
 
 STATS_FIELDS = [f for f, _ in Stats._fields_ if f != "reserved"]
@@ -58,7 +59,8 @@ STATS_FIELDS = [f for f, _ in Stats._fields_ if f != "reserved"]
 class Tuning(C.Structure):
     _fields_ = [("tile", C.c_int * 3), ("block_threads", C.c_int),
                 ("refill_batch", C.c_int), ("chunk_min", C.c_int),
-                ("walk_blocks", C.c_int), ("lds_fp32", C.c_int)]
+                ("walk_blocks", C.c_int), ("lds_fp32", C.c_int),
+                ("finish_packets", C.c_int)]
 
 
 _lib = None
@@ -230,7 +232,7 @@ class Context:
             raise CMIError(rc, self.lib.cmi_last_error(self.h).decode())
 
     def set_engine(self, engine, tile=None, block_threads=0, refill_batch=0,
-                   chunk_min=0, walk_blocks=0, lds_fp32=0):
+                   chunk_min=0, walk_blocks=0, lds_fp32=0, finish_packets=0):
         t = Tuning()
         if tile is not None:
             for a in range(3):
@@ -240,6 +242,7 @@ class Context:
         t.chunk_min = int(chunk_min)
         t.walk_blocks = int(walk_blocks)
         t.lds_fp32 = int(lds_fp32)
+        t.finish_packets = int(finish_packets)
         self._check(self.lib.cmi_set_engine(self.h, int(engine), C.byref(t)))
 
     def set_reemission(self, probability, seed=None, iteration=0):

@@ -148,6 +148,8 @@ void fill_stats(cmi_ctx *c, cmi_stats *st, uint64_t packets, double seconds,
                 uint32_t rounds) {
   if (!st) return;
   st->reemitted = c->h_cnt->reemitted;
+  st->walkout_crossings = 0;  // the walk engine has no walk-out launch
+  st->walkout_hops = 0;
   st->packets = packets;
   st->crossings = c->h_cnt->crossings;
   st->hops = c->h_cnt->hops;

@@ -174,7 +174,8 @@ struct Counters {
   unsigned long long errors;  // invalid exit mask or exceeded walk bound
   unsigned long long next_packet;  // persistent-refill ticket
   unsigned long long reemitted;    // reemission events
-  unsigned long long pad;
+  unsigned long long walkout_crossings;  // of crossings: those of walk-out launches
+  unsigned long long walkout_hops;       // of hops: tile legs of walk-out launches
 };
 
 // ---------------------------------------------------------------------------

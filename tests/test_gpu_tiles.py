@@ -120,8 +120,9 @@ def test_tiles_transport_matches_walk(oracle):
 def test_tiles_round_count_and_timing_switch():
     """The host checks the packets left only every few rounds; the reported
     round count is still the number of rounds that had packets (= the most
-    tile visits of any packet), and switching the per-launch timing off
-    changes nothing but the timing record."""
+    tile visits of any packet, when no round is walked out in one launch),
+    and switching the per-launch timing off changes nothing but the timing
+    record."""
     ncell, nsub = [64, 64, 64], [4, 4, 4]
     box = W.box_10pc()
     n, xn = _field(6, ncell, nsub, -6, -3)
@@ -130,7 +131,7 @@ def test_tiles_round_count_and_timing_switch():
         c64.set_sources(pos, L)
         pk, ss = c64.emit(20000, 2, 0)
     with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
-        ctx.set_engine(cmi.ENGINE_TILES, tile=(8, 8, 8))
+        ctx.set_engine(cmi.ENGINE_TILES, tile=(8, 8, 8), finish_packets=-1)
         ctx.set_fields(n, xn)
         _, st, cr, hp, stats = ctx.transport_packets(pk, ss)
         assert stats["errors"] == 0
@@ -152,6 +153,64 @@ def test_tiles_round_count_and_timing_switch():
         assert np.allclose(J_on, J_off, rtol=1e-12, atol=0)
 
 
+def test_tiles_walkout_matches_visits():
+    """When the packets left (counted at the start and every 8 rounds) are at
+    most finish_packets, one walk-out launch walks them all to the end, with
+    the opacities read from memory.  Each tile leg repeats a visit's
+    arithmetic, so every packet's crossings, tile visits, fate and final
+    position agree exactly with walking it visit by visit, and J agrees to
+    the order of its FP64 sums.  Thresholds: never, the default and 1e9 (the
+    whole transport walked out at once), and N - 1 (walked out at the first
+    check, after 8 rounds)."""
+    ncell, nsub = [64, 64, 64], [4, 4, 4]
+    box = W.box_10pc()
+    n, xn = _field(7, ncell, nsub, -6, -3)
+    pos, L = W.random_sources(4, box, seed=11)
+    N, NE = 200000, 300000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as c64:
+        c64.set_sources(pos, L)
+        pk, ss = c64.emit(N, 2, 0)
+    out = {}
+    for fp in (-1, 0, N - 1, 10 ** 9):
+        with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+            ctx.set_engine(cmi.ENGINE_TILES, tile=(8, 8, 8), finish_packets=fp)
+            ctx.set_fields(n, xn)
+            p, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+            J, _ = ctx.get_fields()
+            ctx.set_sources(pos, L)
+            ctx.reset_intensity()
+            s2 = ctx.transport(NE, 4, 2)
+            J2, _ = ctx.get_fields()
+        out[fp] = (p, st, cr, hp, stats, J, s2, J2)
+    p0, st0, cr0, hp0, s0, J0, e0, Je0 = out[-1]
+    assert s0["errors"] == 0 and s0["rounds"] == int(hp0.max())
+    assert s0["walkout_crossings"] == 0 and e0["walkout_crossings"] == 0
+    # the first check comes after 8 rounds: this medium needs more
+    assert s0["rounds"] > 9 and e0["rounds"] > 9
+    for fp in (0, N - 1, 10 ** 9):
+        p, st, cr, hp, stats, J, s2, J2 = out[fp]
+        assert stats["errors"] == 0, fp
+        assert np.array_equal(st, st0), fp
+        assert np.array_equal(cr, cr0), fp
+        assert np.array_equal(hp, hp0), fp
+        assert np.array_equal(p["pos"], p0["pos"]), fp
+        for k in ("crossings", "absorbed", "escaped", "hops"):
+            assert stats[k] == s0[k], (fp, k)
+            assert s2[k] == e0[k], (fp, k)
+        assert np.allclose(J, J0, rtol=1e-10, atol=0), fp
+        assert np.allclose(J2, Je0, rtol=1e-10, atol=0), fp
+    for fp in (0, 10 ** 9):  # under the threshold from the start
+        stats, s2 = out[fp][4], out[fp][6]
+        assert stats["rounds"] == 1 and s2["rounds"] == 1
+        assert stats["walkout_crossings"] == stats["crossings"]
+        assert stats["walkout_hops"] == stats["hops"]
+    stats, s2 = out[N - 1][4], out[N - 1][6]
+    for s in (stats, s2):  # rounds 0-7, then round 8 walked out
+        assert s["rounds"] == 9
+        assert 0 < s["walkout_crossings"] < s["crossings"]
+        assert 0 < s["walkout_hops"] < s["hops"]
+
+
 def test_tiles_results_independent_of_chunking():
     """How a round's packets are cut into workgroup visits (the chunk, which
     the plan also shrinks for rounds with few packets) changes only the order
@@ -159,7 +218,8 @@ def test_tiles_results_independent_of_chunking():
     fates, tile visits and rounds agree exactly and J to FP64 rounding.
     2e6 packets: the first rounds hold more than a quarter of the resident
     workgroups' worth of full chunks (192 x 8192 on 256 CUs), so the 8192 and
-    16384 chunks are compared uncut there and cut in the later rounds."""
+    16384 chunks are compared uncut there and cut in the later rounds (no
+    round is walked out in one launch here, so every round is cut)."""
     ncell, nsub = [64, 64, 64], [4, 4, 4]
     box = W.box_10pc()
     n, xn = _field(8, ncell, nsub, -6, -3)
@@ -168,7 +228,7 @@ def test_tiles_results_independent_of_chunking():
     N = 2_000_000
     for chunk in (256, 8192, 16384):
         with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
-            ctx.set_engine(cmi.ENGINE_TILES, chunk_min=chunk)
+            ctx.set_engine(cmi.ENGINE_TILES, chunk_min=chunk, finish_packets=-1)
             ctx.set_fields(n, xn)
             ctx.set_sources(pos, L)
             s = ctx.transport(N, 5, 1)
