@@ -400,12 +400,14 @@ def main():
     # (warm-up and timed transports alike), as a kernel trace of the same
     # command does
     ctx.kernel_timing(reset=True)
-    all_cross = all_hops = 0
+    # crossings and tile visits of the process launches (a walk-out launch,
+    # which walks the last few packets of a transport, is another kernel)
+    all_cross = all_hops = wo_cross = 0
     for it in range(args.warmup):
         s = ctx.transport(N, seed, it)
         l1 = ctx.update_ionization(Ltot, N, W.ALPHA_B, form=1)
-        all_cross += s["crossings"]
-        all_hops += s["hops"]
+        all_cross += s["crossings"] - s["walkout_crossings"]
+        all_hops += s["hops"] - s["walkout_hops"]
         log("warmup %d: %.3f s transport, %d crossings, L1 %.3g"
             % (it, s["seconds"], s["crossings"], l1))
     ctx.synchronize()
@@ -416,14 +418,15 @@ def main():
         l1 = ctx.update_ionization(Ltot, N, W.ALPHA_B, form=1)
         cross += s["crossings"]
         hops += s["hops"]
+        all_cross += s["crossings"] - s["walkout_crossings"]
+        all_hops += s["hops"] - s["walkout_hops"]
+        wo_cross += s["walkout_crossings"]
         log("step %d: %.3f s transport, %d crossings, L1 %.3g"
             % (k, s["seconds"], s["crossings"], l1))
     ctx.synchronize()
     t1 = time.perf_counter()
     dt = t1 - t0
     avg_kernel, calls = ctx.kernel_timing()
-    all_cross += cross
-    all_hops += hops
     b_cross = 20 if prec == cmi.PREC_FP32 else 32
     per_launch = (all_cross * b_cross + all_hops * 64) / max(calls, 1)
     achieved = per_launch / avg_kernel
@@ -454,6 +457,7 @@ def main():
             "cells": ncell ** 3,
             "packets_per_step": N,
             "crossings_per_step": cross / args.steps,
+            "walkout_crossings_per_step": wo_cross / args.steps,
         },
         "roofline": {
             "bound": "hbm",
@@ -467,8 +471,9 @@ def main():
             "launches": calls,
             "bytes_model": ("%d B/crossing + 64 B/%s visit, averaged over "
                             "the launches of the warm-up and timed "
-                            "transports" % (b_cross, "subgrid" if args.engine ==
-                                            "walk" else "tile")),
+                            "transports (walk-out launches excluded)"
+                            % (b_cross, "subgrid" if args.engine == "walk"
+                               else "tile")),
             "crossings_per_launch": all_cross / max(calls, 1),
             "visits_per_launch": all_hops / max(calls, 1),
             "avg_kernel_s": avg_kernel,

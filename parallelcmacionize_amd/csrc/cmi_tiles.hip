@@ -60,10 +60,33 @@ struct ProbGeo {
   double unit_scale[3];  // Delta_0 / Delta_a
 };
 
+// Walk state of a straight flight inside a tile, in one 32-bit register:
+// the cell's offset in the tile plus a bias of one axis-0 plane (T1 * T2
+// cells, so that it stays non-negative one step past any face) in the low
+// bits (offmask), and above it one step counter per axis (log2(T_a) + 1 bits
+// at bit fs[a]).  The counter of axis a starts at the cells already behind
+// the packet along its direction (l_a for a positive direction, T_a - 1 - l_a
+// for a negative one) and grows by one per step along a, so the flight leaves
+// the tile exactly when a counter reaches T_a, its top bit (guard).  A step
+// along a adds one increment, (1 << fs[a]) plus the signed cell stride along
+// a: one select chain, one add and one mask test per crossing move the cell,
+// count the step and detect the exit.  The offset field never borrows from or
+// carries into the counters (its width covers one step past either face).
+// Any tile of at most 16384 cells and 64 cells per axis fits.
+struct WalkLayout {
+  uint32_t fs[3];       // bit of each axis' counter
+  uint32_t guard_a[3];  // top bit of each axis' counter
+  uint32_t guard;       // all three
+  uint32_t offmask;     // the biased offset
+  int bias;             // T1 * T2
+  int pad;              // floats staged before the tile's opacities (>= bias)
+};
+
 struct TileGeo {
   int T[3];          // tile size per axis (cells, powers of two)
   int tshift[3];
   int tile_cells;
+  WalkLayout W;
   int ntiles;        // tiles of all problems
   int nprob;
   float reemit_p;    // reemission probability of an absorbed packet (0: off)
@@ -610,37 +633,34 @@ struct AccAtomic<double> {
   __device__ static void add(double *p, float v) { atomicAdd(p, (double)v); }
 };
 
-// Per-axis step counters of a straight flight inside a tile, packed in one
-// register: axis 0 in bits 16-23, axis 1 in bits 8-15, axis 2 in bits 0-7.
-// The field of axis a starts at 64 - T_a plus the cells already behind the
-// packet along its direction (l_a for a positive direction, T_a - 1 - l_a
-// for a negative one) and grows by one per step along a, so the flight
-// leaves the tile exactly when a field reaches 64 (bit 6 of the field, the
-// guard).  One add and one mask test per crossing replace three index
-// updates and three bound tests, and the count of steps taken is the byte
-// sum of the register's growth.
-constexpr uint32_t kStepGuard = 0x404040u;
-constexpr uint32_t kStepInc0 = 1u << 16, kStepInc1 = 1u << 8, kStepInc2 = 1u;
-
-__device__ __forceinline__ uint32_t step_field(int l, float d, int T) {
-  return (uint32_t)(64 - T + (d > 0.f ? l : T - 1 - l));
+// Walk state (WalkLayout) of a flight in local cell (l0, l1, l2) moving
+// along d, and the increment of a step along each axis.
+__device__ __forceinline__ uint32_t walk_state(const WalkLayout &W, int l0, int l1,
+                                               int l2, float d0, float d1, float d2,
+                                               int T0, int T1, int T2) {
+  const uint32_t f0 = (uint32_t)(d0 > 0.f ? l0 : T0 - 1 - l0);
+  const uint32_t f1 = (uint32_t)(d1 > 0.f ? l1 : T1 - 1 - l1);
+  const uint32_t f2 = (uint32_t)(d2 > 0.f ? l2 : T2 - 1 - l2);
+  const uint32_t off = (uint32_t)((l0 * T1 + l1) * T2 + l2 + W.bias);
+  return (f0 << W.fs[0]) | (f1 << W.fs[1]) | (f2 << W.fs[2]) | off;
 }
-__device__ __forceinline__ uint32_t pack_steps(int l0, int l1, int l2, float d0,
-                                               float d1, float d2, int T0,
-                                               int T1, int T2) {
-  return (step_field(l0, d0, T0) << 16) | (step_field(l1, d1, T1) << 8) |
-         step_field(l2, d2, T2);
+__device__ __forceinline__ uint32_t walk_inc(uint32_t fs, int stride, float d) {
+  return (1u << fs) + (uint32_t)(d > 0.f ? stride : -stride);
 }
-// Cell index along an axis from its step field (T_a or -1 once the flight
-// has left the tile along that axis).
-__device__ __forceinline__ int field_index(uint32_t f, float d, int T) {
-  const int dist = (int)f - (64 - T);
-  return d > 0.f ? dist : T - 1 - dist;
+// Step counter of an axis (T_a once the flight has left through it).
+__device__ __forceinline__ uint32_t walk_count(uint32_t q, uint32_t fs, int T) {
+  return (q >> fs) & (2u * (uint32_t)T - 1u);
 }
-// Steps taken between two step registers of one straight flight (the sum of
-// the byte-wise growth; no field ever decreases).
-__device__ __forceinline__ uint32_t steps_between(uint32_t lp, uint32_t lp0) {
-  return __builtin_amdgcn_sad_u8(lp - lp0, 0u, 0u);
+// Cell index along an axis from its counter (T or -1 past the face).
+__device__ __forceinline__ int walk_index(uint32_t f, float d, int T) {
+  return d > 0.f ? (int)f : T - 1 - (int)f;
+}
+// Sum of the three counters: steps taken between two states of one flight
+// are the difference of the sums (no counter ever decreases).
+__device__ __forceinline__ uint32_t walk_sum(const WalkLayout &W, uint32_t q, int T0,
+                                             int T1, int T2) {
+  return walk_count(q, W.fs[0], T0) + walk_count(q, W.fs[1], T1) +
+         walk_count(q, W.fs[2], T2);
 }
 
 // DDA set-up along one axis for a packet at p (cells, tile frame) in cell l
@@ -674,9 +694,17 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     PacketOut po, Counters *__restrict__ cnt,
     unsigned long long *__restrict__ prob_cross, int refill_batch,
     uint32_t *__restrict__ rel_out) {
+  // LDS: W.pad floats of padding, the tile's opacities, its accumulators.
+  // The padding and the accumulators behind the opacities keep the read
+  // ahead of a cell one step past any face in bounds (its value unused).
   extern __shared__ __align__(16) unsigned char smem[];
-  float *kapL = reinterpret_cast<float *>(smem);
-  Acc *accL = reinterpret_cast<Acc *>(smem + sizeof(float) * G.tile_cells);
+  const WalkLayout W = G.W;
+  float *kapL = reinterpret_cast<float *>(smem) + W.pad;
+  Acc *accL = reinterpret_cast<Acc *>(
+      smem + ((sizeof(float) * (size_t)(W.pad + G.tile_cells) + 15) & ~(size_t)15));
+  // indexed by the biased offset of the walk state
+  const float *kapB = kapL - W.bias;
+  Acc *accB = accL - W.bias;
   __shared__ uint32_t s_next, s_surv, s_desc;
   __shared__ uint32_t s_face[6];
   __shared__ uint32_t s_fbase[6];
@@ -758,9 +786,10 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
 
     // per-lane packet state
     bool live = false, pending = false;
-    int off = 0;
-    int so0 = 0, so1 = 0, so2 = 0;  // signed LDS strides along the direction
-    uint32_t lp = 0, lp0 = 0;       // step fields now and at the leg's start
+    uint32_t q = 0;                      // walk state (WalkLayout)
+    uint32_t cur = 0;                    // its biased cell offset (q & offmask)
+    uint32_t inc0 = 0, inc1 = 0, inc2 = 0;  // its increments per axis
+    uint32_t q_sum0 = 0;                 // counters' sum at the leg's start
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
     float tm0 = 0.f, tm1 = 0.f, tm2 = 0.f, td0 = 0.f, td1 = 0.f, td2 = 0.f;
     float t = 0.f, tau = 0.f;
@@ -772,20 +801,21 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       bool moving = false;
       int wl0 = 0, wl1 = 0, wl2 = 0, exit_axis = 0;
       if (pending) {
-        const bool exited = (lp & kStepGuard) != 0u;
+        const bool exited = (q & W.guard) != 0u;
         if (!exited) {
           // absorbed: the partial last step, up to the target optical depth
           // (reemitting kernels deposited it at the absorption and left
           // tau = 0)
           const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
-          AccAtomic<Acc>::add(&accL[off], sa);
+          AccAtomic<Acc>::add(&accB[cur], sa);
           t = t + sa;
         }
-        const uint32_t vis = vis_acc + steps_between(lp, lp0) + (exited ? 0u : 1u);
-        wl0 = field_index((lp >> 16) & 0x7fu, d0, T0);
-        wl1 = field_index((lp >> 8) & 0x7fu, d1, T1);
-        wl2 = field_index(lp & 0x7fu, d2, T2);
-        exit_axis = (lp & (64u << 16)) ? 0 : ((lp & (64u << 8)) ? 1 : 2);
+        const uint32_t vis =
+            vis_acc + (walk_sum(W, q, T0, T1, T2) - q_sum0) + (exited ? 0u : 1u);
+        wl0 = walk_index(walk_count(q, W.fs[0], T0), d0, T0);
+        wl1 = walk_index(walk_count(q, W.fs[1], T1), d1, T1);
+        wl2 = walk_index(walk_count(q, W.fs[2], T2), d2, T2);
+        exit_axis = (q & W.guard_a[0]) ? 0 : ((q & W.guard_a[1]) ? 1 : 2);
         const int gx = ox + wl0, gy = oy + wl1, gz = oz + wl2;
         const bool in_box = (unsigned)gx < (unsigned)PN0 && (unsigned)gy < (unsigned)PN1 &&
                             (unsigned)gz < (unsigned)PN2;
@@ -873,13 +903,13 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           dda_axis(l0, p0, d0, tm0, td0);
           dda_axis(l1, p1, d1, tm1, td1);
           dda_axis(l2, p2, d2, tm2, td2);
-          so0 = d0 > 0.f ? st0 : -st0;
-          so1 = d1 > 0.f ? st1 : -st1;
-          so2 = d2 > 0.f ? 1 : -1;
-          lp = pack_steps(l0, l1, l2, d0, d1, d2, T0, T1, T2);
-          lp0 = lp;
-          off = (l0 * T1 + l1) * T2 + l2;
-          kap_c = kapL[off];
+          inc0 = walk_inc(W.fs[0], st0, d0);
+          inc1 = walk_inc(W.fs[1], st1, d1);
+          inc2 = walk_inc(W.fs[2], 1, d2);
+          q = walk_state(W, l0, l1, l2, d0, d1, d2, T0, T1, T2);
+          q_sum0 = walk_sum(W, q, T0, T1, T2);
+          cur = q & W.offmask;
+          kap_c = kapB[cur];
           live = true;
         }
       }
@@ -909,9 +939,16 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
       const int nlive0 = __popcll(live_mask);
       const int stop_at = nxt_mask ? max(nlive0 - refill_batch, 0) : 0;
       int nlive = nlive0;
+      // two steps per pass: the count of finished lanes (a ballot) and the
+      // loop's exec-mask bookkeeping are paid once per two crossings; a lane
+      // that finishes in the first step sits out the second
+      constexpr int kSteps = kReemit ? 1 : 2;
       while (nlive > stop_at) {
-        bool fin = false;
+        bool fin = false;  // lanes that finished in this pass
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
         if (live) {
+          bool f;
           const float tn = fminf(tm0, fminf(tm1, tm2));
           const float seg = tn - t;
           const float dtau = seg * kap_c;
@@ -920,7 +957,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             bool again = false;
             if (kReemit) {
               const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
-              AccAtomic<Acc>::add(&accL[off], sa);
+              AccAtomic<Acc>::add(&accB[cur], sa);
               t = t + sa;
               tau = 0.f;
               const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
@@ -932,10 +969,10 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
               if (((float)(r.x >> 8) + 0.5f) * k24 < G.reemit_p) {
                 // get_random_direction, testDensitySubGrid_omp.cpp:529-545,
                 // and tau = -ln u: FP32 draws on 24-bit uniforms in (0, 1)
-                const int l0 = field_index((lp >> 16) & 0x7fu, d0, T0);
-                const int l1 = field_index((lp >> 8) & 0x7fu, d1, T1);
-                const int l2 = field_index(lp & 0x7fu, d2, T2);
-                vis_acc += steps_between(lp, lp0) + 1u;
+                const int l0 = walk_index(walk_count(q, W.fs[0], T0), d0, T0);
+                const int l1 = walk_index(walk_count(q, W.fs[1], T1), d1, T1);
+                const int l2 = walk_index(walk_count(q, W.fs[2], T2), d2, T2);
+                vis_acc += (walk_sum(W, q, T0, T1, T2) - q_sum0) + 1u;
                 const float cost = 2.f * (((float)(r.y >> 8) + 0.5f) * k24) - 1.f;
                 const float phi = 6.28318530717958647692f * (((float)(r.z >> 8) + 0.5f) * k24);
                 const float sint = sqrtf(fmaxf(1.f - cost * cost, 0.f));
@@ -954,33 +991,35 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
                 dda_axis(l0, p0, d0, tm0, td0);
                 dda_axis(l1, p1, d1, tm1, td1);
                 dda_axis(l2, p2, d2, tm2, td2);
-                so0 = d0 > 0.f ? st0 : -st0;
-                so1 = d1 > 0.f ? st1 : -st1;
-                so2 = d2 > 0.f ? 1 : -1;
-                lp = pack_steps(l0, l1, l2, d0, d1, d2, T0, T1, T2);
-                lp0 = lp;
+                inc0 = walk_inc(W.fs[0], st0, d0);
+                inc1 = walk_inc(W.fs[1], st1, d1);
+                inc2 = walk_inc(W.fs[2], 1, d2);
+                q = walk_state(W, l0, l1, l2, d0, d1, d2, T0, T1, T2);
+                q_sum0 = walk_sum(W, q, T0, T1, T2);
                 again = true;
               }
             }
-            fin = !again;
+            f = !again;
           } else {
-            AccAtomic<Acc>::add(&accL[off], seg);
+            AccAtomic<Acc>::add(&accB[cur], seg);
             tau -= dtau;
             t = tn;
             const bool a0 = tm0 == tn;
             const bool a1 = !a0 && (tm1 == tn);
-            off += a0 ? so0 : (a1 ? so1 : so2);
-            lp += a0 ? kStepInc0 : (a1 ? kStepInc1 : kStepInc2);
+            q += a0 ? inc0 : (a1 ? inc1 : inc2);
             tm0 += a0 ? td0 : 0.f;
             tm1 += a1 ? td1 : 0.f;
             tm2 += (!a0 && !a1) ? td2 : 0.f;
-            fin = (lp & kStepGuard) != 0u;
-            kap_c = kapL[fin ? 0 : off];  // the next cell, read ahead
+            f = (q & W.guard) != 0u;
+            cur = q & W.offmask;
+            kap_c = kapB[cur];  // the next cell, read ahead
           }
-          if (fin) {
+          if (f) {
             live = false;
             pending = true;
+            fin = true;
           }
+        }
         }
         nlive -= __popcll(__ballot(fin));
       }
@@ -1069,6 +1108,7 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
     PacketOut po, Counters *__restrict__ cnt, int max_tiles) {
   const uint32_t n = total[0];
   const ProbGeo &P = G.probs[0];
+  const WalkLayout W = G.W;
   const int T0 = G.T[0], T1 = G.T[1], T2 = G.T[2];
   const int st0 = T1 * T2, st1 = T2;
   const int PN0 = P.N[0], PN1 = P.N[1], PN2 = P.N[2];
@@ -1099,14 +1139,14 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
         dda_axis(l0, p0, d0, tm0, td0);
         dda_axis(l1, p1, d1, tm1, td1);
         dda_axis(l2, p2, d2, tm2, td2);
-        const int so0 = d0 > 0.f ? st0 : -st0, so1 = d1 > 0.f ? st1 : -st1,
-                  so2 = d2 > 0.f ? 1 : -1;
-        uint32_t lp = pack_steps(l0, l1, l2, d0, d1, d2, T0, T1, T2);
-        const uint32_t lp0 = lp;
-        int off = (l0 * T1 + l1) * T2 + l2;
-        const float *kt = kappa + (size_t)tile * G.tile_cells;
-        double *Jt = J + (size_t)tile * G.tile_cells;
-        float kap_c = kt[off];
+        const uint32_t inc0 = walk_inc(W.fs[0], st0, d0), inc1 = walk_inc(W.fs[1], st1, d1),
+                       inc2 = walk_inc(W.fs[2], 1, d2);
+        uint32_t q = walk_state(W, l0, l1, l2, d0, d1, d2, T0, T1, T2);
+        const uint32_t q_sum0 = walk_sum(W, q, T0, T1, T2);
+        // indexed by the biased offset of the walk state
+        const float *kt = kappa + (size_t)tile * G.tile_cells - W.bias;
+        double *Jt = J + (size_t)tile * G.tile_cells - W.bias;
+        float kap_c = kt[q & W.offmask];
         t = 0.f;
         bool exited = false;
         for (;;) {
@@ -1114,34 +1154,33 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
           const float seg = tn - t;
           const float dtau = seg * kap_c;
           if (dtau >= tau) break;
-          atomicAdd(&Jt[off], (double)seg);
+          atomicAdd(&Jt[q & W.offmask], (double)seg);
           tau -= dtau;
           t = tn;
           const bool a0 = tm0 == tn;
           const bool a1 = !a0 && (tm1 == tn);
-          off += a0 ? so0 : (a1 ? so1 : so2);
-          lp += a0 ? kStepInc0 : (a1 ? kStepInc1 : kStepInc2);
+          q += a0 ? inc0 : (a1 ? inc1 : inc2);
           tm0 += a0 ? td0 : 0.f;
           tm1 += a1 ? td1 : 0.f;
           tm2 += (!a0 && !a1) ? td2 : 0.f;
-          if ((lp & kStepGuard) != 0u) {
+          if ((q & W.guard) != 0u) {
             exited = true;
             break;
           }
-          kap_c = kt[off];
+          kap_c = kt[q & W.offmask];
         }
-        pc_cross += steps_between(lp, lp0) + (exited ? 0u : 1u);
+        pc_cross += (walk_sum(W, q, T0, T1, T2) - q_sum0) + (exited ? 0u : 1u);
         if (!exited) {
           // absorbed: the partial last step, up to the target optical depth
           const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
-          atomicAdd(&Jt[off], (double)sa);
+          atomicAdd(&Jt[q & W.offmask], (double)sa);
           t = t + sa;
           status = 0;
           break;
         }
-        const int e0 = field_index((lp >> 16) & 0x7fu, d0, T0);
-        const int e1 = field_index((lp >> 8) & 0x7fu, d1, T1);
-        const int e2 = field_index(lp & 0x7fu, d2, T2);
+        const int e0 = walk_index(walk_count(q, W.fs[0], T0), d0, T0);
+        const int e1 = walk_index(walk_count(q, W.fs[1], T1), d1, T1);
+        const int e2 = walk_index(walk_count(q, W.fs[2], T2), d2, T2);
         const int nx = ox + e0, ny = oy + e1, nz = oz + e2;
         if (!((unsigned)nx < (unsigned)PN0 && (unsigned)ny < (unsigned)PN1 &&
               (unsigned)nz < (unsigned)PN2)) {
@@ -1149,7 +1188,7 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
           break;
         }
         // the record the next tile's visit would read
-        const int exit_axis = (lp & (64u << 16)) ? 0 : ((lp & (64u << 8)) ? 1 : 2);
+        const int exit_axis = (q & W.guard_a[0]) ? 0 : ((q & W.guard_a[1]) ? 1 : 2);
         const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
         p0 = (exit_axis == 0) ? (d0 > 0.f ? 0.f : (float)T0) : q0;
         p1 = (exit_axis == 1) ? (d1 > 0.f ? 0.f : (float)T1) : q1;
@@ -1383,6 +1422,30 @@ static std::vector<ProbGeo> ctx_problems(const cmi_ctx *c) {
 constexpr int kReemitRoundCap = 100000;
 constexpr size_t kLdsBytes = 160 * 1024;  // LDS of one CU on gfx950
 
+// The walk-state layout of a tile shape (WalkLayout); bits() of the result
+// is at most 32 for every tile set_tile accepts.
+static WalkLayout walk_layout(const int T[3]) {
+  WalkLayout W{};
+  const int cells = T[0] * T[1] * T[2];
+  W.bias = T[1] * T[2];
+  W.pad = (W.bias + 3) & ~3;
+  // biased offsets one step past either face: [0, cells + 2 bias)
+  uint32_t ob = 0;
+  while ((1u << ob) < (uint32_t)(cells + 2 * W.bias)) ++ob;
+  uint32_t w[3];
+  for (int a = 0; a < 3; ++a) {
+    w[a] = 1;
+    while ((1 << (w[a] - 1)) < T[a]) ++w[a];
+  }
+  W.fs[2] = ob;
+  W.fs[1] = ob + w[2];
+  W.fs[0] = ob + w[2] + w[1];
+  for (int a = 0; a < 3; ++a) W.guard_a[a] = (uint32_t)T[a] << W.fs[a];
+  W.guard = W.guard_a[0] | W.guard_a[1] | W.guard_a[2];
+  W.offmask = (1u << ob) - 1u;
+  return W;
+}
+
 static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
   TileGeo G{};
   for (int a = 0; a < 3; ++a) {
@@ -1390,6 +1453,7 @@ static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
     G.tshift[a] = c->g.tshift[a];
   }
   G.tile_cells = c->g.tile_cells;
+  G.W = walk_layout(G.T);
   G.ntiles = e->ntiles;
   G.nprob = e->nprob;
   G.probs = e->d_probs;
@@ -1520,7 +1584,8 @@ static RoundCfg round_cfg(const cmi_ctx *c, const TileGeo &G, size_t n0) {
                 ? std::max<uint32_t>(kChunkFloor, (uint32_t)c->tuning.chunk_min)
                 : kChunk;
   r.fp32acc = c->tuning.lds_fp32 != 0;
-  r.lds = (size_t)G.tile_cells * (sizeof(float) + (r.fp32acc ? sizeof(float) : sizeof(double)));
+  r.lds = ((sizeof(float) * (size_t)(G.W.pad + G.tile_cells) + 15) & ~(size_t)15) +
+          (size_t)G.tile_cells * (r.fp32acc ? sizeof(float) : sizeof(double));
   const int wg_per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (160 * 1024) / (r.lds + 1024)));
   r.grid = c->num_cus * wg_per_cu;
   r.min_visits = std::max<uint32_t>(1, (uint32_t)r.grid / kMinVisitsDiv);
