@@ -336,8 +336,8 @@ def batched_section(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--engine", default="tiles", choices=["walk", "tiles"])
     ap.add_argument("--prec", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--packets", type=float, default=1e8)

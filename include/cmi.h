@@ -406,6 +406,17 @@ int cmi_net_warm_start(cmi_net *net, cmi_ctx *ctx, double luminosity,
 int cmi_net_set_start_rule(cmi_net *net, float logit_cut, double fallback_xn);
 int cmi_net_get_start_rule(const cmi_net *net, float *logit_cut,
                            double *fallback_xn);
+/* Coarse start of the warm start (no reference counterpart: the reference
+ * always starts from x_n = 1e-6, DensitySubGrid.hpp:549).  `coarse` is a
+ * context of the same box whose ncell is an integer factor f >= 2 smaller
+ * per axis, on the same device.  cmi_restrict_density: coarse n_H = the
+ * mean of each f^3 block of the fine n_H, coarse x_n = 1e-6, J = 0, FP32
+ * opacity refreshed.  cmi_prolong_neutral_fraction: fine x_n = the coarse
+ * x_n's logit (clamped to [1e-7, 1 - 1/150]) interpolated trilinearly at
+ * each fine cell centre, J = 0, FP32 opacity refreshed.  Each waits for the
+ * source context's stream and launches on the destination's. */
+int cmi_restrict_density(cmi_ctx *fine, cmi_ctx *coarse);
+int cmi_prolong_neutral_fraction(cmi_ctx *coarse, cmi_ctx *fine);
 /* The same features of a context (by the input mode), as [D][H][W][2] FP32
  * on the host (for building training sets from contexts). */
 int cmi_net_context_features(cmi_net *net, cmi_ctx *ctx, double luminosity,

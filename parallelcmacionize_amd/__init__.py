@@ -151,6 +151,8 @@ SIGNATURES = {
     "cmi_net_warm_start": (C.c_int, [_vp, _vp, C.c_double, C.c_uint64]),
     "cmi_net_context_features": (C.c_int, [_vp, _vp, C.c_double, C.c_uint64,
                                            _fp]),
+    "cmi_restrict_density": (C.c_int, [_vp, _vp]),
+    "cmi_prolong_neutral_fraction": (C.c_int, [_vp, _vp]),
 }
 
 
@@ -372,6 +374,18 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.cmi_synchronize(self.h))
+
+    def restrict_density_to(self, coarse):
+        """coarse (same box, ncell an integer factor smaller) gets this
+        context's n_H averaged over each block, x_n = 1e-6 and J = 0."""
+        rc = self.lib.cmi_restrict_density(self.h, coarse.h)
+        if rc != 0:
+            raise CMIError(rc, self.lib.cmi_last_error(coarse.h).decode())
+
+    def prolong_neutral_fraction_from(self, coarse):
+        """This context's x_n from a coarse context's (trilinear in the
+        clamped logit), J = 0."""
+        self._check(self.lib.cmi_prolong_neutral_fraction(coarse.h, self.h))
 
 
 def write_neutral_fractions_dat(path, box, ncell, nsub, nH, xn):

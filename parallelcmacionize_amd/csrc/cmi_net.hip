@@ -367,6 +367,73 @@ __global__ void ctx_apply_kernel(GridDev g, const float *__restrict__ z, int mod
   }
 }
 
+// Coarse start of the warm start.  restrict: coarse n_H = the mean of each
+// f^3 block of fine cells (the mass of the block), coarse x_n = 1e-6, J = 0,
+// FP32 opacity of that state.
+__global__ void restrict_density_kernel(GridDev gf, const double *__restrict__ nf,
+                                        GridDev gc, int f, int Dc, int Hc, int Wc,
+                                        double *__restrict__ nc, double *__restrict__ xnc,
+                                        double *__restrict__ Jc, float *__restrict__ kappa,
+                                        double kscale) {
+  const size_t nv = (size_t)Dc * Hc * Wc;
+  const double inv = 1. / ((double)f * f * f);
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int cz = (int)(v % Wc), cy = (int)((v / Wc) % Hc), cx = (int)(v / ((size_t)Wc * Hc));
+    double s = 0.;
+    for (int i = 0; i < f; ++i)
+      for (int j = 0; j < f; ++j)
+        for (int k = 0; k < f; ++k)
+          s += nf[lattice_addr(gf, cx * f + i, cy * f + j, cz * f + k)];
+    const size_t a = lattice_addr(gc, cx, cy, cz);
+    const double n = s * inv;
+    nc[a] = n;
+    xnc[a] = 1e-6;
+    Jc[a] = 0.;
+    if (kappa) kappa[a] = (float)(kscale * n * 1e-6);
+  }
+}
+
+// prolong: the fine x_n is the coarse field's logit (clamped like the
+// features) interpolated trilinearly at the fine cell's centre (coarse cell
+// centres at (c + 1/2) f fine cells, the edge cells extended), J = 0, FP32
+// opacity refreshed.
+__global__ void prolong_xn_kernel(GridDev gc, const double *__restrict__ xnc, int f,
+                                  int Dc, int Hc, int Wc, GridDev gf,
+                                  const double *__restrict__ nf, double *__restrict__ xnf,
+                                  double *__restrict__ Jf, float *__restrict__ kappa,
+                                  double kscale) {
+  const int Df = Dc * f, Hf = Hc * f, Wf = Wc * f;
+  const size_t nv = (size_t)Df * Hf * Wf;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
+       v += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(v % Wf), y = (int)((v / Wf) % Hf), x = (int)(v / ((size_t)Wf * Hf));
+    int c0[3], c1[3];
+    double w[3];
+    const int fi[3] = {x, y, z}, nc[3] = {Dc, Hc, Wc};
+    for (int a = 0; a < 3; ++a) {
+      const double u = ((double)fi[a] + 0.5) / (double)f - 0.5;
+      const double fl = floor(u);
+      w[a] = u - fl;
+      c0[a] = min(max((int)fl, 0), nc[a] - 1);
+      c1[a] = min(max((int)fl + 1, 0), nc[a] - 1);
+    }
+    double l = 0.;
+    for (int k = 0; k < 8; ++k) {
+      const int ix = (k & 4) ? c1[0] : c0[0], iy = (k & 2) ? c1[1] : c0[1],
+                iz = (k & 1) ? c1[2] : c0[2];
+      const double wk = ((k & 4) ? w[0] : 1. - w[0]) * ((k & 2) ? w[1] : 1. - w[1]) *
+                        ((k & 1) ? w[2] : 1. - w[2]);
+      l += wk * clamped_logit(xnc[lattice_addr(gc, ix, iy, iz)]);
+    }
+    const size_t a = lattice_addr(gf, x, y, z);
+    const double xn = 1. / (1. + exp(-l));
+    xnf[a] = xn;
+    Jf[a] = 0.;
+    if (kappa) kappa[a] = (float)(kscale * nf[a] * xn);
+  }
+}
+
 __global__ void zero_f32_kernel(float *p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -888,6 +955,61 @@ int cmi_net_warm_start(cmi_net *n, cmi_ctx *c, double luminosity, uint64_t npack
                      c->sigma * c->g.cell_size[0]);
   NET_TRY(n, hipGetLastError());
   NET_TRY(n, hipStreamSynchronize(n->stream));
+  return CMI_OK;
+}
+
+// A coarse context of the same box with ncell f times smaller per axis
+// (f >= 2), on the same device; returns f or 0.
+static int coarse_factor(const cmi_ctx *fine, const cmi_ctx *coarse) {
+  if (!fine || !coarse || fine->device != coarse->device) return 0;
+  const int f = fine->ncell[0] / coarse->ncell[0];
+  if (f < 2) return 0;
+  for (int a = 0; a < 3; ++a) {
+    if (coarse->ncell[a] * f != fine->ncell[a]) return 0;
+    if (fine->box[a] != coarse->box[a] || fine->box[3 + a] != coarse->box[3 + a]) return 0;
+  }
+  return f;
+}
+
+int cmi_restrict_density(cmi_ctx *fine, cmi_ctx *coarse) {
+  const int f = coarse_factor(fine, coarse);
+  if (!f)
+    return coarse ? fail(coarse, CMI_ERR_ARG,
+                         "restrict: the coarse grid must be the fine one's box with "
+                         "ncell an integer factor >= 2 smaller per axis, on its device")
+                  : CMI_ERR_ARG;
+  (void)hipSetDevice(coarse->device);
+  // the fine context's stream has written its fields
+  CMI_TRY_HIP(fine, hipStreamSynchronize(fine->stream));
+  const size_t nv = (size_t)coarse->ncell[0] * coarse->ncell[1] * coarse->ncell[2];
+  const bool fp32 = coarse->precision == CMI_PREC_FP32;
+  hipLaunchKernelGGL(restrict_density_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0,
+                     coarse->stream, fine->g, (const double *)fine->f.n, coarse->g, f,
+                     coarse->ncell[0], coarse->ncell[1], coarse->ncell[2], coarse->f.n,
+                     coarse->f.xn, coarse->f.J, fp32 ? coarse->f.kappa : (float *)nullptr,
+                     coarse->sigma * coarse->g.cell_size[0]);
+  CMI_TRY_HIP(coarse, hipGetLastError());
+  return CMI_OK;
+}
+
+int cmi_prolong_neutral_fraction(cmi_ctx *coarse, cmi_ctx *fine) {
+  const int f = coarse_factor(fine, coarse);
+  if (!f)
+    return fine ? fail(fine, CMI_ERR_ARG,
+                       "prolong: the coarse grid must be the fine one's box with "
+                       "ncell an integer factor >= 2 smaller per axis, on its device")
+                : CMI_ERR_ARG;
+  (void)hipSetDevice(fine->device);
+  CMI_TRY_HIP(coarse, hipStreamSynchronize(coarse->stream));
+  const size_t nv = (size_t)fine->ncell[0] * fine->ncell[1] * fine->ncell[2];
+  const bool fp32 = fine->precision == CMI_PREC_FP32;
+  hipLaunchKernelGGL(prolong_xn_kernel, dim3(grid_for(nv, 256, 8192)), dim3(256), 0,
+                     fine->stream, coarse->g, (const double *)coarse->f.xn, f,
+                     coarse->ncell[0], coarse->ncell[1], coarse->ncell[2], fine->g,
+                     (const double *)fine->f.n, fine->f.xn, fine->f.J,
+                     fp32 ? fine->f.kappa : (float *)nullptr,
+                     fine->sigma * fine->g.cell_size[0]);
+  CMI_TRY_HIP(fine, hipGetLastError());
   return CMI_OK;
 }
 

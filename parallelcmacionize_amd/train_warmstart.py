@@ -9,7 +9,10 @@ Training problems are 256^3 clumpy media like the 256^3 clumpy workload
 plain iteration (1e8 packets per iteration, common random numbers, mean
 |dx_n| < 2e-4) and cut into 64 crops of 64^3.  Inputs (mode 1, the default):
 log10 n_H and the logit of x_n after `passes` cheap iterations of
-`pass_packets` packets; the target is the correction to that logit.
+`pass_packets` packets (start "passes", the default and the shipped
+network's) or of the coarse start's x_n (start "coarse": the medium
+converged on a grid `coarse_factor` times coarser, interpolated back); the
+target is the correction to that logit.
 Training: Adam on whole 64^3 crops, BF16 MFMA kernels with FP32
 accumulation, all on the GPU, with predictions more neutral than the target
 weighted `w_neutral` in the loss.  Then held-out 256^3 problems (seeds in
@@ -54,16 +57,22 @@ def main(argv=None):
                     help="0: inputs log n_H, log jH of the last cheap pass; "
                          "1: log n_H and the cheap iterate's logit x_n, the "
                          "network adds a correction to that logit")
+    ap.add_argument("--start", choices=("coarse", "passes"), default="passes")
+    ap.add_argument("--coarse-factor", type=int, default=WS.COARSE)
+    ap.add_argument("--coarse-packets", type=float, default=WS.COARSE_PACKETS)
+    ap.add_argument("--coarse-tol", type=float, default=WS.COARSE_TOL)
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     npk = int(a.pass_packets)
     N = int(a.packets)
 
     t0 = time.time()
+    coarse = (a.coarse_factor, int(a.coarse_packets), a.coarse_tol)
     feats, tgts, recs = WS.make_training_set(range(a.problems), N, crop=a.crop,
                                              ncell=a.ncell, log=log,
                                              passes=a.passes, first_packets=npk,
-                                             mode=a.mode)
+                                             mode=a.mode, start=a.start,
+                                             coarse=coarse)
     log("training set: %d crops of %d^3 in %.1f s" % (len(feats), a.crop,
                                                       time.time() - t0))
     # FP64 statistics: a float32 reduction over 1e8 values drifts badly
@@ -73,9 +82,11 @@ def main(argv=None):
     net.set_loss_weight(a.w_neutral)
     net.set_input_mode(a.mode)
     net.passes, net.pass_packets = a.passes, npk
+    net.start = a.start
+    net.coarse_factor, net.coarse_packets, net.coarse_tol = coarse
     if a.mode == 1:
-        # the correction starts at zero: the warm start begins as the cheap
-        # iterate itself
+        # the correction starts at zero: the warm start begins as its input
+        # (the coarse solve or the cheap iterate) itself
         p = net.params()
         p[WS.LAST_LAYER:] = 0.0
         net.set_params(p)
@@ -151,6 +162,7 @@ def main(argv=None):
     meta = {"problems": a.problems, "packets": a.packets, "ncell": a.ncell,
             "crop": a.crop, "crops": a.problems * (a.ncell // a.crop) ** 3,
             "passes": a.passes, "pass_packets": npk, "steps": a.steps,
+            "start": a.start, "coarse": list(coarse),
             "batch": a.batch, "train_crop": a.train_crop, "lr": a.lr,
             "w_neutral": a.w_neutral, "train_seconds": t_train,
             "final_train_loss": float(losses[-50:].mean()),

@@ -3,11 +3,20 @@
 The plain method (testDensitySubGrid_omp.cpp:2274-2434) starts every cell at
 x_n = 1e-6 (DensitySubGrid.hpp:549) and alternates a photon transport with the
 ionization-equilibrium update until the neutral fraction stops changing.  The
-warm start replaces the starting x_n only.  The shipped network (input mode
-1) reads the state after PASSES cheap iterations of PASS_PACKETS packets
-(log10 n_H and the logit of that x_n) and adds a correction to that logit;
-the plain iteration then runs from there and still decides convergence (mean
-|x_n change| per cell < tol).  Input mode 0, the first design, reads one
+warm start replaces the starting x_n only; the plain iteration then runs from
+there and still decides convergence (mean |x_n change| per cell < tol).
+
+The shipped network (input mode 1, start "passes") reads log10 n_H and the
+logit of x_n after PASSES cheap iterations of PASS_PACKETS packets on the
+full grid and adds a correction to that logit.  The other start ("coarse")
+reads the logit of a coarse solution instead: the same medium averaged onto
+a grid COARSE times coarser per axis and converged there by the plain
+iteration (COARSE_PACKETS packets, about 1/64 of a full iteration's work per
+iteration), interpolated back to the full grid on the device.  It places
+the fronts of the whole box, but the clumps it averages away leave errors
+the full iteration removes slowly: on the 256^3 clumpy workload it starts
+10-12 iterations from convergence (the cheap passes: 7), 8-9 with a network
+trained on it.  Input mode 0, the first design, reads one
 cheap transport through the x_n = 1e-6 medium instead (log10 n_H and log10 of
 the unattenuated photoionization rate jH) and predicts the logit itself; it
 cannot see the shadows that decide where the fronts are.
@@ -32,6 +41,13 @@ FIRST_PASS_PACKETS = 1_000_000
 # regions, which a single pass through the x_n = 1e-6 medium cannot
 PASSES = 5
 PASS_PACKETS = 10_000_000
+# coarse start: grid factor, packets per coarse iteration (about the full
+# run's crossings per cell: 1e8 packets on 256^3 cross a cell ~680 times),
+# convergence bar of the coarse plain iteration
+COARSE = 4
+COARSE_PACKETS = 3_000_000
+COARSE_TOL = 1e-3
+COARSE_MAX_IT = 40
 XN_CLAMP = 1e-7          # targets use logit of x_n clamped to [1e-7, 1 - 1/150]
 XN_HIGH = 1.0 - 1.0 / 150.0   # above it every cell is opaque (tau ~ 76 per cell)
 LOGIT_SCALE = 8.0        # the network outputs logit / 8
@@ -70,6 +86,8 @@ class Problem:
     def __init__(self, box, n_lattice, pos, L, nsub=4, alpha=W.ALPHA_B):
         ncell = list(n_lattice.shape)
         self.ncell = ncell
+        self.box, self.pos, self.Lsrc = box, pos, L
+        self.coarse = None
         self.nsub = [nsub] * 3
         self.ctx = Context(box, ncell, self.nsub, PREC_FP32)
         self.ctx.set_engine(ENGINE_TILES)
@@ -87,6 +105,41 @@ class Problem:
         self.ctx.set_fields(self.n_ref, self.xn0)
         self.ctx.reset_intensity()
         self.ctx.synchronize()
+
+    def coarse_context(self, factor=COARSE, npackets=COARSE_PACKETS):
+        """The coarse grid of the coarse start (built once, with its
+        transport buffers for npackets, then kept)."""
+        if self.coarse is None or self.coarse[0] != factor or self.coarse[2] < npackets:
+            if self.coarse is not None:
+                self.coarse[1].close()
+            nc = [c // factor for c in self.ncell]
+            ns = [max(1, min(4, c // 16)) for c in nc]
+            ctx = Context(self.box, nc, ns, PREC_FP32)
+            ctx.set_engine(ENGINE_TILES)
+            ctx.set_kernel_timing(False)
+            ctx.set_sources(self.pos, self.Lsrc)
+            ctx.transport(npackets, 0, 0)   # builds the engine's buffers
+            ctx.reset_intensity()
+            self.coarse = (factor, ctx, npackets)
+        return self.coarse[1]
+
+    def coarse_start(self, seed, factor=COARSE, npackets=COARSE_PACKETS,
+                     tol=COARSE_TOL, max_it=COARSE_MAX_IT):
+        """x_n of this context from a coarse solve, all on the device: the
+        density averaged onto the coarse grid, the plain iteration there with
+        common random numbers until mean |dx_n| < tol (from the second
+        iteration on), the coarse logit interpolated back.  Returns the
+        coarse iterations."""
+        c = self.coarse_context(factor, npackets)
+        self.ctx.restrict_density_to(c)
+        it = 0
+        for it in range(1, max_it + 1):
+            c.transport(npackets, seed, 1)
+            l1 = c.update_ionization(self.L, npackets, self.alpha, form=1)
+            if it >= MIN_IT and l1 < tol:
+                break
+        self.ctx.prolong_neutral_fraction_from(c)
+        return it
 
     def xn_lattice(self):
         _, x = self.ctx.get_fields()
@@ -126,6 +179,9 @@ class Problem:
                 "converged": l1s[-1] < tol}
 
     def close(self):
+        if self.coarse is not None:
+            self.coarse[1].close()
+            self.coarse = None
         self.ctx.close()
 
 
@@ -146,19 +202,29 @@ def converge_plain(prob, npackets, seed, tol=1e-3, max_it=40, frozen=True):
     return r
 
 
-def converge_warm(prob, net, npackets, seed, tol=1e-3, max_it=40,
-                  first_packets=None, frozen=True, passes=None):
-    """First pass(es) + network + plain iterations; the wall time includes
-    the first passes and the inference."""
-    if first_packets is None:
-        first_packets = getattr(net, "pass_packets", FIRST_PASS_PACKETS)
-    if passes is None:
-        passes = getattr(net, "passes", 1)
-    prob.reset()
-    t0 = time.perf_counter()
-    prob.first_pass(seed ^ 0x5A5A5A5A, first_packets, passes,
+def net_start(prob, net, seed):
+    """The network's inputs in the context (its coarse solve or its cheap
+    passes); returns the packet count the inputs were made with."""
+    if getattr(net, "start", "passes") == "coarse":
+        prob.coarse_start(seed, net.coarse_factor, net.coarse_packets,
+                          net.coarse_tol)
+        return net.coarse_packets
+    first_packets = getattr(net, "pass_packets", FIRST_PASS_PACKETS)
+    prob.first_pass(seed, first_packets, getattr(net, "passes", 1),
                     update_last=net.input_mode() == 1)
-    net.warm_start(prob.ctx, prob.L, first_packets)
+    return first_packets
+
+
+def converge_warm(prob, net, npackets, seed, tol=1e-3, max_it=40, frozen=True):
+    """The network's start (coarse solve or cheap passes) + inference + plain
+    iterations; the wall time includes the start and the inference."""
+    prob.reset()
+    if getattr(net, "start", "passes") == "coarse":
+        # set-up, like the full grid's context: not part of the timed start
+        prob.coarse_context(net.coarse_factor, net.coarse_packets)
+    t0 = time.perf_counter()
+    npk = net_start(prob, net, seed ^ 0x5A5A5A5A)
+    net.warm_start(prob.ctx, prob.L, npk)
     t1 = time.perf_counter()
     r = prob.iterate(npackets, seed, tol=tol, max_it=max_it, frozen=frozen,
                      min_it=MIN_IT)
@@ -181,7 +247,8 @@ def crops(a, size):
 def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
                       nsrc=8, tol=2e-4, max_it=40, extra_it=0, log=None,
                       feature_net=None, passes=1,
-                      first_packets=FIRST_PASS_PACKETS, mode=0):
+                      first_packets=FIRST_PASS_PACKETS, mode=0, start="passes",
+                      coarse=(COARSE, COARSE_PACKETS, COARSE_TOL)):
     """Converge one clumpy problem per seed with the plain method and cut
     (features, target logits) crops.  Returns float32 arrays
     (S, crop, crop, crop, 2) and (S, crop, crop, crop) plus per-problem
@@ -194,9 +261,13 @@ def make_training_set(seeds, npackets, crop=64, ncell=256, box_side_pc=10.0,
                                         field_seed=1000 + s, src_seed=2000 + s,
                                         lum_seed=3000 + s)
         prob = Problem(box, n, pos, L)
-        prob.first_pass(seed=7000 + s, npackets=first_packets, passes=passes,
-                        update_last=mode == 1)
-        f = net.context_features(prob.ctx, prob.L, first_packets)
+        if start == "coarse":
+            prob.coarse_start(7000 + s, *coarse)
+            f = net.context_features(prob.ctx, prob.L, coarse[1])
+        else:
+            prob.first_pass(seed=7000 + s, npackets=first_packets, passes=passes,
+                            update_last=mode == 1)
+            f = net.context_features(prob.ctx, prob.L, first_packets)
         prob.reset()
         r = prob.iterate(npackets, seed=5000 + s, tol=tol, max_it=max_it,
                          frozen=True, min_it=MIN_IT)
@@ -245,6 +316,10 @@ def save_net(net, path, meta=None, passes=1, pass_packets=FIRST_PASS_PACKETS):
     mean, scale = getattr(net, "norm", (FEATURE_MEAN, FEATURE_SCALE))
     np.savez(path, params=net.params(), feature_mean=np.asarray(mean),
              passes=passes, pass_packets=pass_packets,
+             start=np.array(getattr(net, "start", "passes")),
+             coarse_factor=getattr(net, "coarse_factor", COARSE),
+             coarse_packets=getattr(net, "coarse_packets", COARSE_PACKETS),
+             coarse_tol=getattr(net, "coarse_tol", COARSE_TOL),
              input_mode=net.input_mode(),
              feature_scale=np.asarray(scale), logit_scale=LOGIT_SCALE,
              dilation=net.dilation(),
@@ -265,5 +340,9 @@ def load_net(path):
         net.set_input_mode(int(d["input_mode"]))
     net.passes = int(d["passes"]) if "passes" in d else 1
     net.pass_packets = int(d["pass_packets"]) if "pass_packets" in d else FIRST_PASS_PACKETS
+    net.start = str(d["start"]) if "start" in d else "passes"
+    net.coarse_factor = int(d["coarse_factor"]) if "coarse_factor" in d else COARSE
+    net.coarse_packets = int(d["coarse_packets"]) if "coarse_packets" in d else COARSE_PACKETS
+    net.coarse_tol = float(d["coarse_tol"]) if "coarse_tol" in d else COARSE_TOL
     net.set_params(d["params"])
     return net

@@ -80,6 +80,44 @@ def test_warm_start_writes_network_prediction(small):
     assert mism.sum() <= 8
 
 
+def test_coarse_start_restrict_and_prolong():
+    """The coarse start's device steps against numpy on a 64^3 grid and its
+    16^3 coarsening: the coarse n_H is the mean of each 4^3 block (read back
+    through the network's mode-1 features, log10 n_H and the logit of x_n =
+    1e-6), and the fine x_n is the coarse logit (clamped) interpolated
+    trilinearly at the fine cell centres, the edge cells extended."""
+    box = W.box_10pc()
+    rng = np.random.default_rng(3)
+    nf = 1e8 * np.exp(rng.normal(0, 1, (64, 64, 64)))
+    xc = 10 ** rng.uniform(-7.5, 0, (16, 16, 16))
+    net = WS.new_net(5)
+    net.set_input_mode(1)
+    with cmi.Context(box, [64] * 3, [4] * 3, cmi.PREC_FP32) as fine, \
+            cmi.Context(box, [16] * 3, [1] * 3, cmi.PREC_FP32) as coarse:
+        fine.set_fields(W.global_to_subgrid_major(nf, [4] * 3), np.full(64 ** 3, 1e-6))
+        fine.restrict_density_to(coarse)
+        f = net.context_features(coarse, 1.0, 1).reshape(16, 16, 16, 2)
+        coarse.set_fields(xn=W.global_to_subgrid_major(xc, [1] * 3))
+        fine.prolong_neutral_fraction_from(coarse)
+        J, xf = fine.get_fields()
+        xf = W.subgrid_major_to_global(xf, [64] * 3, [4] * 3)
+    net.close()
+    n_blk = nf.reshape(16, 4, 16, 4, 16, 4).mean(axis=(1, 3, 5))
+    assert np.allclose(f[..., 0], np.log10(n_blk), rtol=0, atol=2e-6)
+    assert np.allclose(f[..., 1], np.log(1e-6 / (1 - 1e-6)), rtol=1e-6)
+    # trilinear = three 1-D linear interpolations
+    u = (np.arange(64) + 0.5) / 4 - 0.5
+    i0 = np.floor(u).astype(int)
+    w = u - i0
+    c0, c1 = np.clip(i0, 0, 15), np.clip(i0 + 1, 0, 15)
+    lg = WS.logit(xc)
+    lg = lg[c0] * (1 - w)[:, None, None] + lg[c1] * w[:, None, None]
+    lg = lg[:, c0] * (1 - w)[None, :, None] + lg[:, c1] * w[None, :, None]
+    lg = lg[:, :, c0] * (1 - w)[None, None, :] + lg[:, :, c1] * w[None, None, :]
+    assert np.allclose(xf, 1 / (1 + np.exp(-lg)), rtol=1e-12, atol=0)
+    assert not J.any()
+
+
 @pytest.mark.skipif(not os.path.exists(NET), reason="no trained network file")
 def test_trained_network_gain_on_256_clumpy():
     """The committed network on the 256^3 clumpy workload (never trained on):
