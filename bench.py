@@ -518,6 +518,11 @@ def main():
         # what binds: the larger of the measured HBM and VALU-issue fractions
         R["limiter"] = ("valu_issue" if R["issue"]["frac"] > R["traffic_frac"]
                         else "hbm")
+        R["limiter_note"] = (
+            "the larger of the two measured fractions; neither is saturated: "
+            "occupancy is capped at 6 waves per SIMD by LDS (three 49 KiB "
+            "workgroups per CU) and the crossing loop is a chain of "
+            "dependent LDS reads and FP64 LDS atomics")
     xn_now = ctx.get_fields()[1]
     if headline and not args.no_error:
         out["error_vs_fp64"] = error_vs_fp64(box, ncell, nsub, n, xn_now, pos, L,
