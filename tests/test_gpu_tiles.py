@@ -23,7 +23,8 @@ def _field(seed, ncell, nsub, xlo=-6, xhi=-3):
     return n, xn
 
 
-@pytest.mark.parametrize("tile", [(16, 16, 16), (8, 8, 8), (4, 8, 16)])
+@pytest.mark.parametrize("tile", [(16, 16, 16), (8, 8, 8), (4, 8, 16),
+                                  (1, 16, 16), (16, 2, 4), (2, 16, 1)])
 def test_tiles_list_matches_fp64_oracle(oracle, tile):
     ncell, nsub = [32, 32, 32], [2, 2, 2]
     box = W.box_10pc()
@@ -104,6 +105,39 @@ def test_tiles_transport_matches_walk(oracle):
             ctx.set_fields(n, xn)
             ctx.set_sources(pos, L)
             s = ctx.transport(N, 3, 1)
+            J, _ = ctx.get_fields()
+            out[eng] = (s, J)
+    sw, Jw = out[cmi.ENGINE_WALK]
+    st, Jt = out[cmi.ENGINE_TILES]
+    assert st["errors"] == 0 and st["absorbed"] + st["escaped"] == N
+    assert abs(st["absorbed"] - sw["absorbed"]) <= max(5, 1e-4 * N)
+    assert abs(st["crossings"] / sw["crossings"] - 1) < 2e-3
+    assert abs(Jt.sum() / Jw.sum() - 1) < 1e-5
+    big = Jw > 1e-3 * Jw.max()
+    rel = np.abs(Jt[big] - Jw[big]) / Jw[big]
+    assert np.median(rel) < 1e-5
+
+
+@pytest.mark.parametrize("tile", [(64, 8, 4), (4, 64, 16), (2, 4, 64)])
+def test_tiles_wide_tiles_match_walk(tile):
+    """Tiles with a 64-cell axis (the widest step counter of the packed walk
+    state) on a grid of 64^3 subgrids: the same intensity field and fates
+    as the walk engine, within the FP32 bars."""
+    ncell, nsub = [128, 128, 128], [2, 2, 2]
+    box = W.box_10pc()
+    n, xn = _field(9, ncell, nsub, -5, -2)
+    pos, L = W.random_sources(4, box, seed=13)
+    N = 300000
+    out = {}
+    for eng in (cmi.ENGINE_WALK, cmi.ENGINE_TILES):
+        with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+            if eng == cmi.ENGINE_TILES:
+                ctx.set_engine(eng, tile=tile)
+            else:
+                ctx.set_engine(eng)
+            ctx.set_fields(n, xn)
+            ctx.set_sources(pos, L)
+            s = ctx.transport(N, 5, 1)
             J, _ = ctx.get_fields()
             out[eng] = (s, J)
     sw, Jw = out[cmi.ENGINE_WALK]
