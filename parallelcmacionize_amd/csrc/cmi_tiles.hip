@@ -190,12 +190,24 @@ __device__ __forceinline__ int set_emit(const SetSources &S, uint64_t kg,
                                         uint32_t key1, double pos[3],
                                         double dir[3], double *tau, int *src,
                                         uint64_t *rank) {
-  int p = 0;
-  while (p + 1 < S.nprob && kg >= S.pk_cum[p]) ++p;
+  // the problem: the first p with kg < pk_cum[p] (the last one if none);
+  // the source: the first s of the problem with k < src_cum[s] (likewise).
+  // Binary searches: a problem set has up to 64 problems of up to 16
+  // sources, and every step of a search is a dependent load
+  int lo = 0, hi = S.nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (kg < S.pk_cum[mid]) hi = mid; else lo = mid + 1;
+  }
+  const int p = lo;
   const uint64_t k = kg - (p ? S.pk_cum[p - 1] : 0);
-  int s = S.src_begin[p];
-  const int s_end = S.src_begin[p + 1];
-  while (s + 1 < s_end && k >= S.src_cum[s]) ++s;
+  lo = S.src_begin[p];
+  hi = S.src_begin[p + 1] - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (k < S.src_cum[mid]) hi = mid; else lo = mid + 1;
+  }
+  const int s = lo;
   const u32x4 r0 = philox4x32_10(
       u32x4{(uint32_t)k, (uint32_t)(k >> 32) ^ ((uint32_t)p << 20), iteration, 0u},
       key0, key1);
@@ -274,10 +286,19 @@ __global__ void tiles_source_hist_kernel(TileGeo G,
                                          const SetSources *__restrict__ S,
                                          uint32_t *__restrict__ hist,
                                          SrcTile *__restrict__ src_tile) {
-  for (int p = 0; p < S->nprob; ++p) {
+  // one thread per source of every problem; a source's problem is the last
+  // p with src_begin[p] <= s
+  const int ns = S->src_begin[S->nprob];
+  for (int s = (int)threadIdx.x; s < ns; s += blockDim.x) {
+    int lo = 0, hi = S->nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (S->src_begin[mid] <= s) lo = mid; else hi = mid - 1;
+    }
+    const int p = lo;
     const ProbGeo &P = G.probs[p];
-    const int b = S->src_begin[p], e = S->src_begin[p + 1];
-    for (int s = b + (int)threadIdx.x; s < e; s += blockDim.x) {
+    const int b = S->src_begin[p];
+    {
       const uint64_t c = S->src_cum[s] - (s > b ? S->src_cum[s - 1] : 0);
       int g[3];
       double pc[3];
@@ -427,11 +448,27 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
   const int per = (ntiles + kPlanThreads - 1) / kPlanThreads;
   const int t0 = threadIdx.x * per;
   const int t1 = min(ntiles, t0 + per);
+  // each thread's tile counts are read once, into registers, when it has at
+  // most kPlanPer tiles (ntiles <= 16384): the three passes below then make
+  // no dependent global loads (a set of 64 problems has ~11000 tiles)
+  constexpr int kPlanPer = 16;
+  const bool cached = per <= kPlanPer;
+  uint32_t hc[kPlanPer];
+  if (cached) {
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j) hc[j] = (t0 + j < t1) ? hist[t0 + j] : 0u;
+  }
   uint64_t cnt = 0;
-  for (int t = t0; t < t1; ++t) {
-    const uint32_t c = hist[t];
+  auto pass1 = [&](int t, uint32_t c) {
     cnt += c;
     if (track && c) atomicAdd(&s_pc[tile_prob[t]], c);
+  };
+  if (cached) {
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j)
+      if (t0 + j < t1) pass1(t0 + j, hc[j]);
+  } else {
+    for (int t = t0; t < t1; ++t) pass1(t, hist[t]);
   }
   uint64_t tot_c, tot_d;
   uint64_t base_c = block_exclusive_scan(cnt, sh, &tot_c);
@@ -440,10 +477,15 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     chunk = (uint32_t)(want < chunk ? (want > kChunkFloor ? want : kChunkFloor) : chunk);
   }
   uint64_t nch = 0;
-  for (int t = t0; t < t1; ++t) nch += (hist[t] + chunk - 1) / chunk;
+  if (cached) {
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j)
+      if (t0 + j < t1) nch += (hc[j] + chunk - 1) / chunk;
+  } else {
+    for (int t = t0; t < t1; ++t) nch += (hist[t] + chunk - 1) / chunk;
+  }
   uint64_t base_d = block_exclusive_scan(nch, sh, &tot_d);
-  for (int t = t0; t < t1; ++t) {
-    const uint32_t c = hist[t];
+  auto pass3 = [&](int t, uint32_t c) {
     seg[t] = (uint32_t)base_c;
     cursor[t] = 0;
     hist[t] = 0;
@@ -464,6 +506,13 @@ __global__ void __launch_bounds__(kPlanThreads) tiles_plan_kernel(
     }
     base_d += nch;
     base_c += c;
+  };
+  if (cached) {
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j)
+      if (t0 + j < t1) pass3(t0 + j, hc[j]);
+  } else {
+    for (int t = t0; t < t1; ++t) pass3(t, hist[t]);
   }
   __syncthreads();
   const uint32_t ncrowded = min(s_ncrowded, (uint32_t)kPlanCrowded);
