@@ -369,6 +369,10 @@ int cmi_net_get_dilation(const cmi_net *net, int dilation[4]);
  * (e.g. after a few cheap iterations) and the output is added to it. */
 int cmi_net_set_input_mode(cmi_net *net, int mode);
 int cmi_net_get_input_mode(const cmi_net *net);
+/* Forward convolutions through the LDS-tiled kernel where the volume divides
+ * into its blocks (default 1), or always through the per-run kernel (0).
+ * Both give identical outputs; the switch is there to check that. */
+int cmi_net_set_tiled(cmi_net *net, int on);
 /* weight of the squared logit error where the prediction is more neutral
  * than the target (default 1: plain mean squared error) */
 int cmi_net_set_loss_weight(cmi_net *net, float w_too_neutral);

@@ -138,6 +138,7 @@ SIGNATURES = {
     "cmi_net_set_dilation": (C.c_int, [_vp, _ip]),
     "cmi_net_set_loss_weight": (C.c_int, [_vp, C.c_float]),
     "cmi_net_set_input_mode": (C.c_int, [_vp, C.c_int]),
+    "cmi_net_set_tiled": (C.c_int, [_vp, C.c_int]),
     "cmi_net_get_input_mode": (C.c_int, [_vp]),
     "cmi_net_set_start_rule": (C.c_int, [_vp, C.c_float, C.c_double]),
     "cmi_net_get_start_rule": (C.c_int, [_vp, C.POINTER(C.c_float),
@@ -569,6 +570,12 @@ class WarmStartNet:
 
     def set_input_mode(self, mode):
         self._check(self.lib.cmi_net_set_input_mode(self.h, int(mode)))
+
+    def set_tiled(self, on):
+        """Forward convolutions through the LDS-tiled kernel where the
+        volume divides into its blocks (default), or always through the
+        per-run kernel; the outputs are identical."""
+        self._check(self.lib.cmi_net_set_tiled(self.h, int(bool(on))))
 
     def input_mode(self):
         return self.lib.cmi_net_get_input_mode(self.h)

@@ -117,6 +117,135 @@ __global__ void __launch_bounds__(256) conv3_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Tiled forward convolution, for the shapes it divides: the same outputs as
+// conv3_kernel bit for bit (every output voxel's 27 taps go through the same
+// MFMAs with the same operands in the same order).  A convolution with
+// dilation d is d^3 ordinary 3x3x3 convolutions on the sub-lattices of voxels
+// with equal coordinates mod d.  A workgroup takes a block of one sub-lattice,
+// kTZ x kTY runs of 16 voxels along the last axis, and stages the block with
+// its one-voxel halo in LDS, so that each input voxel is read from memory
+// once per block instead of once per tap and output run; each wave computes
+// kRunsPerWave runs and loads each tap's weights once for all of them.
+// ---------------------------------------------------------------------------
+constexpr int kTZ = 4, kTY = 8;
+constexpr int kTiledWaves = 4;
+constexpr int kRunsPerWave = kTZ * kTY / kTiledWaves;
+constexpr int kHZ = kTZ + 2, kHY = kTY + 2, kHX = 16 + 2;  // block + halo (voxels)
+
+__host__ __device__ inline bool tiled_shape_ok(int D, int H, int W, int dil) {
+  return dil >= 1 && W % (16 * dil) == 0 && H % (kTY * dil) == 0 && D % (kTZ * dil) == 0;
+}
+
+template <int NB>
+__global__ void __launch_bounds__(64 * kTiledWaves) conv3_tiled_kernel(
+    const __bf16 *__restrict__ x, const __bf16 *__restrict__ w,
+    const float *__restrict__ bias, int D, int H, int W, int dil,
+    __bf16 *__restrict__ y, float *__restrict__ yf, int relu) {
+  __shared__ __align__(16) __bf16 blk[kHZ * kHY * kHX * kC];
+  const int Ds = D / dil, Hs = H / dil, Ws = W / dil;  // sub-lattice extents
+  const int nbz = Ds / kTZ, nby = Hs / kTY, nbx = Ws / 16;
+  int t = blockIdx.x;
+  const int bx = t % nbx;
+  t /= nbx;
+  const int by = t % nby;
+  t /= nby;
+  const int bz = t % nbz;
+  t /= nbz;
+  const int rx = t % dil;
+  t /= dil;
+  const int ry = t % dil;
+  t /= dil;
+  const int rz = t % dil;
+  const int b = t / dil;
+  // the block with its halo, in sub-lattice coordinates from (z0, y0, x0);
+  // out of the volume is the zero padding of conv3_kernel
+  const int z0 = bz * kTZ - 1, y0 = by * kTY - 1, x0 = bx * 16 - 1;
+  // every thread issues all of its loads before its first LDS store, so the
+  // block's memory latency is paid once, not once per load
+  constexpr int kQuarters = kHZ * kHY * kHX * 4;
+  constexpr int kThreadsT = 64 * kTiledWaves;
+  constexpr int kLoads = (kQuarters + kThreadsT - 1) / kThreadsT;
+  bf16x8 vals[kLoads];
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) {
+    const int i = (int)threadIdx.x + k * kThreadsT;
+    vals[k] = bf16x8{};
+    if (i < kQuarters) {
+      const int q = i & 3;  // 16-byte quarter of the voxel's 32 channels
+      const int v = i >> 2;
+      const int hx = v % kHX, hy = (v / kHX) % kHY, hz = v / (kHX * kHY);
+      const int sz = z0 + hz, sy = y0 + hy, sx = x0 + hx;
+      if ((unsigned)sz < (unsigned)Ds && (unsigned)sy < (unsigned)Hs &&
+          (unsigned)sx < (unsigned)Ws) {
+        const size_t vox = (((size_t)b * D + (rz + dil * sz)) * H + (ry + dil * sy)) * W +
+                           (rx + dil * sx);
+        vals[k] = *reinterpret_cast<const bf16x8 *>(x + vox * kC + 8 * q);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kLoads; ++k) {
+    const int i = (int)threadIdx.x + k * kThreadsT;
+    if (i < kQuarters) *reinterpret_cast<bf16x8 *>(blk + (size_t)i * 8) = vals[k];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int vl = lane & 15;  // voxel (A row) / output channel (B column)
+  const int kg = lane >> 4;  // k group: channels 8*kg .. 8*kg+7
+  const int NOUT = 16 * NB;
+  f32x4 acc[kRunsPerWave][NB];
+#pragma unroll
+  for (int r = 0; r < kRunsPerWave; ++r)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[r][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // each tap's weights are loaded one tap ahead
+  bf16x8 bw[NB], bn[NB];
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb)
+    bn[nb] = *reinterpret_cast<const bf16x8 *>(w + ((size_t)nb * 16 + vl) * kC + 8 * kg);
+#pragma unroll 1
+  for (int tap = 0; tap < kTaps; ++tap) {
+    const int kd = tap / 9, kh = (tap / 3) % 3, kw = tap % 3;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bw[nb] = bn[nb];
+    if (tap + 1 < kTaps) {
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        bn[nb] = *reinterpret_cast<const bf16x8 *>(
+            w + ((size_t)(tap + 1) * NOUT + nb * 16 + vl) * kC + 8 * kg);
+    }
+#pragma unroll
+    for (int r = 0; r < kRunsPerWave; ++r) {
+      const int j = wave * kRunsPerWave + r;
+      const int jz = j / kTY, jy = j % kTY;
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(
+          blk + ((size_t)((jz + kd) * kHY + (jy + kh)) * kHX + (vl + kw)) * kC + 8 * kg);
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nb], acc[r][nb], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kRunsPerWave; ++r) {
+    const int j = wave * kRunsPerWave + r;
+    const int gz = rz + dil * (bz * kTZ + (j / kTY)), gy = ry + dil * (by * kTY + (j % kTY));
+    const size_t row = (((size_t)b * D + gz) * H + gy) * W;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = nb * 16 + vl;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const size_t v = row + rx + dil * (bx * 16 + 4 * kg + i);
+        float o = acc[r][nb][i] + (bias ? bias[co] : 0.f);
+        if (relu) o = fmaxf(o, 0.f);
+        if (y) y[v * kC + co] = f2bf(o);
+        if (yf && co == 0) yf[v] = o;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: dW[tap][cout][cin] += sum over voxels of
 // dy[v][cout] * x[v + tap - 1][cin].  A wave owns (tap, cout block) and both
 // cin blocks and sweeps `rows_per_wave` runs of 32 voxels.
@@ -472,6 +601,7 @@ struct cmi_net {
   int dil[kLayers] = {1, 1, 1, 1};  // dilation of each layer's 3x3x3 taps
   float w_pos = 1.f;  // loss weight of predictions more neutral than the target
   int mode = 0;       // input mode of the warm start (0: jH, 1: refine x_n)
+  int tiled = 1;      // forward convolutions through conv3_tiled_kernel where it fits
   // dataset (device): features [S][N][2] FP32, targets [S][N] FP32
   float *ds_feat = nullptr, *ds_tgt = nullptr;
   int ds_n = 0, ds_dim[3] = {0, 0, 0};
@@ -536,6 +666,20 @@ int forward(cmi_net *n, int B, int D, int H, int W, float *zout) {
   const unsigned blocks = grid_for((size_t)nrun, 4, 1u << 30);  // one wave per run
   for (int l = 0; l < kLayers; ++l) {
     const float *bias = n->p + n->off_b[l];
+    const int dil = n->dil[l];
+    if (n->tiled && tiled_shape_ok(D, H, W, dil)) {
+      // the same outputs, blocks of a dilation sub-lattice staged in LDS
+      const unsigned tb = (unsigned)((size_t)B * D * H * W / (16 * kTZ * kTY));
+      if (l < kLayers - 1)
+        hipLaunchKernelGGL(conv3_tiled_kernel<2>, dim3(tb), dim3(64 * kTiledWaves), 0,
+                           n->stream, n->act[l], n->wf[l], bias, D, H, W, dil, n->act[l + 1],
+                           (float *)nullptr, 1);
+      else
+        hipLaunchKernelGGL(conv3_tiled_kernel<1>, dim3(tb), dim3(64 * kTiledWaves), 0,
+                           n->stream, n->act[l], n->wf[l], bias, D, H, W, dil,
+                           (__bf16 *)nullptr, zout, 0);
+      continue;
+    }
     if (l < kLayers - 1) {
       hipLaunchKernelGGL(conv3_kernel<2>, dim3(blocks), dim3(256), 0, n->stream, n->act[l],
                          n->wf[l], bias, nrun, D, H, W, n->dil[l], n->act[l + 1], (float *)nullptr,
@@ -756,6 +900,12 @@ int cmi_net_set_input_mode(cmi_net *n, int mode) {
 }
 
 int cmi_net_get_input_mode(const cmi_net *n) { return n ? n->mode : -1; }
+
+int cmi_net_set_tiled(cmi_net *n, int on) {
+  if (!n) return CMI_ERR_ARG;
+  n->tiled = on ? 1 : 0;
+  return CMI_OK;
+}
 
 int cmi_net_set_loss_weight(cmi_net *n, float w_too_neutral) {
   if (!n || !(w_too_neutral > 0.f)) return CMI_ERR_ARG;
