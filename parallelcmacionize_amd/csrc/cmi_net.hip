@@ -225,6 +225,38 @@ __global__ void __launch_bounds__(64 * kTiledWaves) conv3_tiled_kernel(
         acc[r][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[nb], acc[r][nb], 0, 0, 0);
     }
   }
+  if (NB == 2 && y && !yf) {
+    // 32-channel output: through the (now free) LDS block as [run][16
+    // voxels][32 channels], then written out in 16-byte pieces, a voxel's
+    // 64 bytes by 4 neighbouring lanes
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRunsPerWave; ++r) {
+      const int j = wave * kRunsPerWave + r;
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int co = nb * 16 + vl;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float o = acc[r][nb][i] + (bias ? bias[co] : 0.f);
+          if (relu) o = fmaxf(o, 0.f);
+          blk[((size_t)j * 16 + 4 * kg + i) * kC + co] = f2bf(o);
+        }
+      }
+    }
+    __syncthreads();
+    constexpr int kOut = kTZ * kTY * 16 * 4;  // 16-byte pieces of the block's outputs
+#pragma unroll
+    for (int k = 0; k < kOut / kThreadsT; ++k) {
+      const int i = (int)threadIdx.x + k * kThreadsT;
+      const int q = i & 3, vx = (i >> 2) & 15, j = i >> 6;
+      const int gz = rz + dil * (bz * kTZ + (j / kTY)), gy = ry + dil * (by * kTY + (j % kTY));
+      const size_t v = (((size_t)b * D + gz) * H + gy) * W + rx + dil * (bx * 16 + vx);
+      *reinterpret_cast<bf16x8 *>(y + v * kC + 8 * q) =
+          *reinterpret_cast<const bf16x8 *>(blk + (size_t)i * 8);
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < kRunsPerWave; ++r) {
     const int j = wave * kRunsPerWave + r;
