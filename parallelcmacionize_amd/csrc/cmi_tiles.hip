@@ -32,6 +32,10 @@
 // lanes is free (or the wave runs dry) and then write out and reload
 // together, so the divergent refill path is paid once per batch rather than
 // once per packet.
+//
+// The last packets of a single transport, once a host check (at the start
+// and every kRoundsPerCheck rounds) finds at most one per resident lane, are
+// walked to their ends by one walk-out launch instead of further rounds.
 #include <float.h>
 #include <string.h>
 
@@ -1610,7 +1614,7 @@ static int ensure_engine(cmi_ctx *c, const std::vector<ProbGeo> &probs,
 struct RoundCfg {
   int threads, refill, grid;
   uint32_t chunk, min_visits;
-  uint32_t finish_max;  // a round with at most this many packets finishes in one launch
+  uint32_t finish_max;  // walk-out threshold: packets left at a host check (0: never)
   bool fp32acc;
   size_t lds;
   unsigned scatter_blocks;
