@@ -118,6 +118,39 @@ def test_coarse_start_restrict_and_prolong():
     assert not J.any()
 
 
+def test_coarse_start_fp64_and_refusals():
+    """The coarse start on FP64 contexts (no opacity array) gives the same
+    fields as on FP32 ones, and grids that are not an integer coarsening of
+    the same box are refused with CMI_ERR_ARG, leaving the fields alone."""
+    box = W.box_10pc()
+    rng = np.random.default_rng(4)
+    nf = 1e8 * np.exp(rng.normal(0, 1, (32, 32, 32)))
+    xc = 10 ** rng.uniform(-7, 0, (8, 8, 8))
+    out = {}
+    for prec in (cmi.PREC_FP32, cmi.PREC_FP64):
+        with cmi.Context(box, [32] * 3, [2] * 3, prec) as fine, \
+                cmi.Context(box, [8] * 3, [1] * 3, prec) as coarse:
+            fine.set_fields(W.global_to_subgrid_major(nf, [2] * 3), np.full(32 ** 3, 1e-6))
+            fine.restrict_density_to(coarse)
+            coarse.set_fields(xn=W.global_to_subgrid_major(xc, [1] * 3))
+            fine.prolong_neutral_fraction_from(coarse)
+            out[prec] = fine.get_fields()[1]
+    assert np.array_equal(out[cmi.PREC_FP32], out[cmi.PREC_FP64])
+    other = np.array(box, dtype=float)
+    other[3:] *= 2.0                       # same cells, another box
+    with cmi.Context(box, [32] * 3, [2] * 3, cmi.PREC_FP32) as fine, \
+            cmi.Context(box, [12] * 3, [1] * 3, cmi.PREC_FP32) as c12, \
+            cmi.Context(other, [8] * 3, [1] * 3, cmi.PREC_FP32) as c_box, \
+            cmi.Context(box, [32] * 3, [2] * 3, cmi.PREC_FP32) as same:
+        fine.set_fields(W.global_to_subgrid_major(nf, [2] * 3), np.full(32 ** 3, 0.5))
+        for c in (c12, c_box, same):
+            with pytest.raises(cmi.CMIError):
+                fine.restrict_density_to(c)
+            with pytest.raises(cmi.CMIError):
+                fine.prolong_neutral_fraction_from(c)
+        assert np.all(fine.get_fields()[1] == 0.5)
+
+
 @pytest.mark.skipif(not os.path.exists(NET), reason="no trained network file")
 def test_trained_network_gain_on_256_clumpy():
     """The committed network on the 256^3 clumpy workload (never trained on):
