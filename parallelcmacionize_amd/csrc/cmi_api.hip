@@ -411,6 +411,10 @@ int cmi_set_engine(cmi_ctx *c, int engine, const cmi_tuning *t) {
 int cmi_set_fields(cmi_ctx *c, const double *nH, const double *xn) {
   if (!c) return CMI_ERR_ARG;
   (void)hipSetDevice(c->device);
+  // the copies below go through the null stream, which does not wait for
+  // the context's non-blocking stream: work still queued there (a
+  // restriction or prolongation, a warm start) must not land after them
+  CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<double> d;
   if (nH) {
     to_device_order(c, nH, d);
@@ -433,6 +437,7 @@ int cmi_set_fields(cmi_ctx *c, const double *nH, const double *xn) {
 int cmi_set_intensity(cmi_ctx *c, const double *J) {
   if (!c || !J) return CMI_ERR_ARG;
   (void)hipSetDevice(c->device);
+  CMI_TRY_HIP(c, hipStreamSynchronize(c->stream));  // as in cmi_set_fields
   std::vector<double> d;
   to_device_order(c, J, d);
   if (c->precision == CMI_PREC_FP32) {

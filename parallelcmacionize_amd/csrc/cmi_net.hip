@@ -860,6 +860,7 @@ int cmi_net_get_params(cmi_net *n, float *out) {
 int cmi_net_set_params(cmi_net *n, const float *in) {
   if (!n || !in) return CMI_ERR_ARG;
   (void)hipSetDevice(n->device);
+  NET_TRY(n, hipStreamSynchronize(n->stream));  // null-stream copies below
   NET_TRY(n, hipMemcpy(n->p, in, n->nparam * sizeof(float), hipMemcpyHostToDevice));
   NET_TRY(n, hipMemset(n->m, 0, n->nparam * sizeof(float)));
   NET_TRY(n, hipMemset(n->v, 0, n->nparam * sizeof(float)));
@@ -885,6 +886,7 @@ int cmi_net_add_samples(cmi_net *n, const float *feat, const float *target, int 
                         const int dim[3]) {
   if (!n || !feat || !target || nsamples <= 0 || !dim) return CMI_ERR_ARG;
   (void)hipSetDevice(n->device);
+  NET_TRY(n, hipStreamSynchronize(n->stream));  // null-stream copies below
   if (n->ds_n > 0 && (dim[0] != n->ds_dim[0] || dim[1] != n->ds_dim[1] || dim[2] != n->ds_dim[2]))
     return nfail(n, CMI_ERR_ARG, "all samples must have the same shape");
   const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
@@ -1030,6 +1032,7 @@ int cmi_net_loss_and_grad(cmi_net *n, const float *feat, const float *target, in
   if (!n || !feat || !target || B <= 0 || !dim) return CMI_ERR_ARG;
   if (dim[2] % 32 != 0) return nfail(n, CMI_ERR_ARG, "last extent must be a multiple of 32");
   (void)hipSetDevice(n->device);
+  NET_TRY(n, hipStreamSynchronize(n->stream));  // null-stream copies below
   const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
   int rc = ensure_work(n, nv * B);
   if (rc != CMI_OK) return rc;
@@ -1068,6 +1071,7 @@ int cmi_net_predict(cmi_net *n, const float *feat, const int dim[3], float *logi
   if (!n || !feat || !dim || !logits) return CMI_ERR_ARG;
   if (dim[2] % 32 != 0) return nfail(n, CMI_ERR_ARG, "last extent must be a multiple of 32");
   (void)hipSetDevice(n->device);
+  NET_TRY(n, hipStreamSynchronize(n->stream));  // null-stream copies below
   const size_t nv = (size_t)dim[0] * dim[1] * dim[2];
   int rc = ensure_work(n, nv);
   if (rc != CMI_OK) return rc;
