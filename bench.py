@@ -263,7 +263,7 @@ def precision_section(args):
     """The uniform Stromgren sphere at 128^3 (8^3 subgrids of 16^3), 1e8
     packets per iteration, run two ways from x_n = 1e-6 with the same
     packets: the all-FP64 GPU walk (the reference's arithmetic) and the FP32
-    tiles engine with FP64 deposits.  Crossings/s of each, and the FP32
+    tiles engine (fixed-point deposits summed into the FP64 J).  Crossings/s of each, and the FP32
     run's neutral-fraction error against the FP64 one and its ionized radius
     against the analytic Stromgren radius."""
     import parallelcmacionize_amd as cmi
@@ -529,9 +529,10 @@ def main():
                         else "hbm")
         R["limiter_note"] = (
             "the larger of the two measured fractions; neither is saturated: "
-            "occupancy is capped at 6 waves per SIMD by LDS (three 49 KiB "
-            "workgroups per CU) and the crossing loop is a chain of "
-            "dependent LDS reads and FP64 LDS atomics")
+            "the crossing loop is a chain of dependent LDS accesses (an "
+            "opacity read and a 64-bit integer add per crossing) on the LDS "
+            "the CU's waves share; occupancy is capped at 6 waves per SIMD "
+            "by LDS, and more resident waves measured no faster")
     xn_now = ctx.get_fields()[1]
     if headline and not args.no_error:
         out["error_vs_fp64"] = error_vs_fp64(box, ncell, nsub, n, xn_now, pos, L,

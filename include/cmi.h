@@ -44,7 +44,8 @@ extern "C" {
  * DensitySubGrid::interact. */
 #define CMI_PREC_FP64 0
 /* FP32 geometry and optical depth (cell units, kappa = sigma n x_n stored as
- * FP32), FP64 deposits into J. */
+ * FP32); path lengths summed per tile visit in 64-bit fixed point (2^-30 of
+ * a cell for cubic cells, exact integer sums) and added into the FP64 J. */
 #define CMI_PREC_FP32 1
 
 /* ---- transport engines ---- */
@@ -97,13 +98,13 @@ typedef struct {
                         such chunks uses smaller ones (down to 256) */
   int walk_blocks;   /* walk: persistent grid size */
   int lds_fp32;      /* tiles: 1 = FP32 LDS accumulators for J (flushed into
-                        FP64), 0 = FP64 */
+                        FP64; slower on gfx950), 0 = 64-bit fixed point */
   int finish_packets;  /* tiles: when the packets left (counted at the start
                           and every 8 rounds) are at most this many, one
                           walk-out launch walks them all to the end
-                          (opacities read from memory, FP64 atomic
-                          deposits; the same cells, crossings and fates as
-                          the rounds).  0 = the default threshold, < 0 =
+                          (opacities read from memory, the visits'
+                          quantized deposits added by FP64 atomics; the
+                          same cells, crossings and fates as the rounds).  0 = the default threshold, < 0 =
                           never.  Single contexts without reemission only. */
 } cmi_tuning;
 
