@@ -1122,9 +1122,21 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
         s_fbase[threadIdx.x] = base;
       }
       __syncthreads();
-      for (uint32_t i = threadIdx.x; i < ns; i += kThreads) {
-        const uint32_t v = rel_out[D.start + i];
-        rel_out[D.start + i] = s_fbase[v >> 29] + (v & ((1u << 29) - 1u));
+      // eight loads in flight per thread, then their rewrites: a loop of
+      // one load, wait and store per slot pays a memory latency per slot
+      constexpr int kFix = 8;
+      for (uint32_t i0 = threadIdx.x; i0 < ns; i0 += kFix * kThreads) {
+        uint32_t v[kFix];
+#pragma unroll
+        for (int k = 0; k < kFix; ++k) {
+          const uint32_t i = i0 + (uint32_t)k * kThreads;
+          v[k] = i < ns ? rel_out[D.start + i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kFix; ++k) {
+          const uint32_t i = i0 + (uint32_t)k * kThreads;
+          if (i < ns) rel_out[D.start + i] = s_fbase[v[k] >> 29] + (v[k] & ((1u << 29) - 1u));
+        }
       }
       if (prob_cross) {
         // crossings of this visit, per problem (one atomic per wave)
