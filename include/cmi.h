@@ -87,7 +87,7 @@ typedef struct {
 /* Tuning of the engines (0 = default for any field). */
 typedef struct {
   int tile[3];       /* LDS tile in cells (powers of two up to 64 dividing
-                        the subgrid) */
+                        the subgrid, at most 8192 cells) */
   int block_threads; /* tiles: threads per workgroup (256 or 512) */
   int refill_batch;  /* tiles: finished lanes a wave gathers before it writes
                         them out and reloads together (1..64) */

@@ -81,15 +81,15 @@ int default_tile(int v, int cap) {
 int set_tile(cmi_ctx *c, const int tile[3]) {
   for (int a = 0; a < 3; ++a) {
     const int s = ilog2_exact(tile[a]);
-    // at most 64 cells per axis and 16384 per tile: the tiles engine keeps a
-    // flight's cell and per-axis step counts in one 32-bit word (an LDS tile
-    // of opacities and accumulators holds fewer cells anyway)
+    // at most 64 cells per axis and 8192 per tile: the tiles engine keeps a
+    // flight's cell (as a byte offset) and per-axis step counts in one
+    // 32-bit word
     if (s < 0 || tile[a] > c->sc[a] || c->sc[a] % tile[a] != 0 || tile[a] > 64)
       return fail(c, CMI_ERR_ARG,
                   "tile sizes must be powers of two up to 64 dividing the subgrid");
   }
-  if (tile[0] * tile[1] * tile[2] > 16384)
-    return fail(c, CMI_ERR_ARG, "at most 16384 cells per tile");
+  if (tile[0] * tile[1] * tile[2] > 8192)
+    return fail(c, CMI_ERR_ARG, "at most 8192 cells per tile");
   for (int a = 0; a < 3; ++a) {
     c->tile[a] = tile[a];
     c->g.tshift[a] = ilog2_exact(tile[a]);

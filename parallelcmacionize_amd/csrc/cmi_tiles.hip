@@ -66,25 +66,27 @@ struct ProbGeo {
 };
 
 // Walk state of a straight flight inside a tile, in one 32-bit register:
-// the cell's offset in the tile plus a bias of one axis-0 plane (T1 * T2
-// cells, so that it stays non-negative one step past any face) in the low
-// bits (offmask), and above it one step counter per axis (log2(T_a) + 1 bits
-// at bit fs[a]).  The counter of axis a starts at the cells already behind
+// the byte offset of the cell's FP32 opacity, 4 (cell + bias), with a bias
+// of one axis-0 plane (T1 * T2 cells rounded up to a multiple of 4, so that
+// it stays non-negative one step past any face) in the low bits (offmask),
+// and above it one step counter per axis (log2(T_a) + 1 bits at bit fs[a]).
+// Byte offsets let a visit read the opacity at the offset itself (the
+// opacities' biased base is the start of the LDS block).  The counter of axis a starts at the cells already behind
 // the packet along its direction (l_a for a positive direction, T_a - 1 - l_a
 // for a negative one) and grows by one per step along a, so the flight leaves
 // the tile exactly when a counter reaches T_a, its top bit (guard).  A step
-// along a adds one increment, (1 << fs[a]) plus the signed cell stride along
+// along a adds one increment, (1 << fs[a]) plus the signed byte stride along
 // a: one select chain, one add and one mask test per crossing move the cell,
 // count the step and detect the exit.  The offset field never borrows from or
 // carries into the counters (its width covers one step past either face).
-// Any tile of at most 16384 cells and 64 cells per axis fits.
+// Any tile of at most 8192 cells and 64 cells per axis fits.
 struct WalkLayout {
   uint32_t fs[3];       // bit of each axis' counter
   uint32_t guard_a[3];  // top bit of each axis' counter
   uint32_t guard;       // all three
-  uint32_t offmask;     // the biased offset
-  int bias;             // T1 * T2
-  int pad;              // floats staged before the tile's opacities (>= bias)
+  uint32_t offmask;     // the biased byte offset
+  int bias;             // T1 * T2 rounded up to a multiple of 4
+  int pad;              // floats staged before the tile's opacities (= bias)
 };
 
 struct TileGeo {
@@ -705,6 +707,29 @@ struct AccAtomic<unsigned long long> {
   __device__ static double value(unsigned long long v, double inv) { return (double)v * inv; }
 };
 
+// Walk layout of a T0 x T1 x T2 tile (a constant expression for the
+// kernels specialised to one tile shape).
+constexpr __host__ __device__ WalkLayout walk_layout(int T0, int T1, int T2) {
+  WalkLayout W{};
+  const int cells = T0 * T1 * T2;
+  W.bias = (T1 * T2 + 3) & ~3;
+  W.pad = W.bias;
+  // biased byte offsets one step past either face: [0, 4 (cells + 2 bias))
+  uint32_t ob = 2;
+  while ((1u << ob) < 4u * (uint32_t)(cells + 2 * W.bias)) ++ob;
+  const int T[3] = {T0, T1, T2};
+  uint32_t w[3] = {1, 1, 1};
+  for (int a = 0; a < 3; ++a)
+    while ((1 << (w[a] - 1)) < T[a]) ++w[a];
+  W.fs[2] = ob;
+  W.fs[1] = ob + w[2];
+  W.fs[0] = ob + w[2] + w[1];
+  for (int a = 0; a < 3; ++a) W.guard_a[a] = (uint32_t)T[a] << W.fs[a];
+  W.guard = W.guard_a[0] | W.guard_a[1] | W.guard_a[2];
+  W.offmask = (1u << ob) - 1u;
+  return W;
+}
+
 // Walk state (WalkLayout) of a flight in local cell (l0, l1, l2) moving
 // along d, and the increment of a step along each axis.
 __device__ __forceinline__ uint32_t walk_state(const WalkLayout &W, int l0, int l1,
@@ -713,7 +738,7 @@ __device__ __forceinline__ uint32_t walk_state(const WalkLayout &W, int l0, int 
   const uint32_t f0 = (uint32_t)(d0 > 0.f ? l0 : T0 - 1 - l0);
   const uint32_t f1 = (uint32_t)(d1 > 0.f ? l1 : T1 - 1 - l1);
   const uint32_t f2 = (uint32_t)(d2 > 0.f ? l2 : T2 - 1 - l2);
-  const uint32_t off = (uint32_t)((l0 * T1 + l1) * T2 + l2 + W.bias);
+  const uint32_t off = (uint32_t)((l0 * T1 + l1) * T2 + l2 + W.bias) << 2;
   return (f0 << W.fs[0]) | (f1 << W.fs[1]) | (f2 << W.fs[2]) | off;
 }
 __device__ __forceinline__ uint32_t walk_inc(uint32_t fs, int stride, float d) {
@@ -755,7 +780,7 @@ __device__ __forceinline__ void dda_axis(int l, float p, float d, float &tm,
 // cell.  The draws are Philox4x32-10 keyed by the transport's seed and
 // iteration, with the absorption point's coordinate bits, the tile and the
 // packet's reemission count as the counter.
-template <class Acc, int kThreads, bool kList, bool kReemit = false>
+template <class Acc, int kThreads, bool kList, bool kReemit = false, int kTile = 0>
 __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     TileGeo G, const float *__restrict__ kappa, double *__restrict__ J,
     const Desc *__restrict__ desc, const uint32_t *__restrict__ ndesc_ptr,
@@ -769,15 +794,36 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   // LDS: W.pad floats of padding, the tile's opacities, its accumulators.
   // The padding and the accumulators behind the opacities keep the read
   // ahead of a cell one step past any face in bounds (its value unused).
-  extern __shared__ __align__(16) unsigned char smem[];
-  const WalkLayout W = G.W;
+  // kTile > 0: the kernel serves kTile^3 tiles only, from a static block
+  // whose address the compiler folds into the LDS instructions' offsets;
+  // kTile = 0: any shape, dynamic LDS.
+  constexpr int kCellsF = kTile * kTile * kTile;
+  constexpr WalkLayout kWF = walk_layout(kTile ? kTile : 1, kTile ? kTile : 1, kTile ? kTile : 1);
+  constexpr size_t kStatBytes =
+      kTile ? ((sizeof(float) * (size_t)(kWF.pad + kCellsF) + 15) & ~(size_t)15) +
+                  sizeof(Acc) * (size_t)kCellsF
+            : 16;
+  __shared__ __align__(16) unsigned char sblk[kStatBytes];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  unsigned char *const smem = kTile ? sblk : dyn;
+  const WalkLayout W = kTile ? kWF : G.W;
+  const int tile_cells = kTile ? kCellsF : G.tile_cells;
+  // the offset mask with its sign bit known clear, so that the compiler may
+  // fold the LDS base into the instruction's offset field
+  const uint32_t offmask = W.offmask & 0x7fffffffu;
   const float fscale = G.fix_scale;
   float *kapL = reinterpret_cast<float *>(smem) + W.pad;
   Acc *accL = reinterpret_cast<Acc *>(
-      smem + ((sizeof(float) * (size_t)(W.pad + G.tile_cells) + 15) & ~(size_t)15));
-  // indexed by the biased offset of the walk state
-  const float *kapB = kapL - W.bias;
-  Acc *accB = accL - W.bias;
+      smem + ((sizeof(float) * (size_t)(W.pad + tile_cells) + 15) & ~(size_t)15));
+  // addressed by the biased byte offset of the walk state: the opacities'
+  // biased base is smem itself (pad == bias), an accumulator sits at
+  // sizeof(Acc) / 4 times the offset from its biased base
+  const unsigned char *kapB = smem;
+  unsigned char *accB = reinterpret_cast<unsigned char *>(accL - W.bias);
+  auto kap_at = [&](uint32_t o) { return *reinterpret_cast<const float *>(kapB + o); };
+  auto acc_at = [&](uint32_t o) {
+    return reinterpret_cast<Acc *>(accB + o * (uint32_t)(sizeof(Acc) / sizeof(float)));
+  };
   __shared__ uint32_t s_next, s_surv, s_desc;
   __shared__ uint32_t s_face[6];
   __shared__ uint32_t s_fbase[6];
@@ -786,8 +832,9 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t ndesc = *ndesc_ptr;
-  const int T0 = G.T[0], T1 = G.T[1], T2 = G.T[2];
-  const int st0 = T1 * T2, st1 = T2;
+  const int T0 = kTile ? kTile : G.T[0], T1 = kTile ? kTile : G.T[1],
+            T2 = kTile ? kTile : G.T[2];
+  const int st0 = (T1 * T2) << 2, st1 = T2 << 2;  // byte strides
   // per-lane counters of this launch (32 bits: a lane walks at most a few
   // hundred thousand crossings per launch); widened for the final sums
   uint32_t my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0, my_err = 0;
@@ -814,11 +861,11 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     // stage the tile
     {
       const float4 *src =
-          reinterpret_cast<const float4 *>(kappa + (size_t)tile * G.tile_cells);
+          reinterpret_cast<const float4 *>(kappa + (size_t)tile * tile_cells);
       float4 *dst = reinterpret_cast<float4 *>(kapL);
-      for (int i = threadIdx.x; i < (G.tile_cells >> 2); i += kThreads)
+      for (int i = threadIdx.x; i < (tile_cells >> 2); i += kThreads)
         dst[i] = src[i];
-      for (int i = threadIdx.x; i < G.tile_cells; i += kThreads) accL[i] = (Acc)0;
+      for (int i = threadIdx.x; i < tile_cells; i += kThreads) accL[i] = (Acc)0;
       if (threadIdx.x == 0) {
         s_next = kThreads;  // the first kThreads packets are claimed below
         s_surv = 0;
@@ -860,7 +907,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     // per-lane packet state
     bool live = false, pending = false;
     uint32_t q = 0;                      // walk state (WalkLayout)
-    uint32_t cur = 0;                    // its biased cell offset (q & offmask)
+    uint32_t cur = 0;                    // its biased byte offset (q & offmask)
     uint32_t inc0 = 0, inc1 = 0, inc2 = 0;  // its increments per axis
     uint32_t q_sum0 = 0;                 // counters' sum at the leg's start
     float p0 = 0.f, p1 = 0.f, p2 = 0.f, d0 = 0.f, d1 = 0.f, d2 = 0.f;
@@ -880,7 +927,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           // (reemitting kernels deposited it at the absorption and left
           // tau = 0)
           const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
-          AccAtomic<Acc>::add(&accB[cur], sa, fscale);
+          AccAtomic<Acc>::add(acc_at(cur), sa, fscale);
           t = t + sa;
         }
         const uint32_t vis =
@@ -978,11 +1025,11 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
           dda_axis(l2, p2, d2, tm2, td2);
           inc0 = walk_inc(W.fs[0], st0, d0);
           inc1 = walk_inc(W.fs[1], st1, d1);
-          inc2 = walk_inc(W.fs[2], 1, d2);
+          inc2 = walk_inc(W.fs[2], 4, d2);
           q = walk_state(W, l0, l1, l2, d0, d1, d2, T0, T1, T2);
           q_sum0 = walk_sum(W, q, T0, T1, T2);
-          cur = q & W.offmask;
-          kap_c = kapB[cur];
+          cur = q & offmask;
+          kap_c = kap_at(cur);
           live = true;
         }
       }
@@ -1030,7 +1077,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             bool again = false;
             if (kReemit) {
               const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
-              AccAtomic<Acc>::add(&accB[cur], sa, fscale);
+              AccAtomic<Acc>::add(acc_at(cur), sa, fscale);
               t = t + sa;
               tau = 0.f;
               const float q0 = fmaf(t, d0, p0), q1 = fmaf(t, d1, p1), q2 = fmaf(t, d2, p2);
@@ -1066,7 +1113,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
                 dda_axis(l2, p2, d2, tm2, td2);
                 inc0 = walk_inc(W.fs[0], st0, d0);
                 inc1 = walk_inc(W.fs[1], st1, d1);
-                inc2 = walk_inc(W.fs[2], 1, d2);
+                inc2 = walk_inc(W.fs[2], 4, d2);
                 q = walk_state(W, l0, l1, l2, d0, d1, d2, T0, T1, T2);
                 q_sum0 = walk_sum(W, q, T0, T1, T2);
                 again = true;
@@ -1074,7 +1121,7 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             }
             f = !again;
           } else {
-            AccAtomic<Acc>::add(&accB[cur], seg, fscale);
+            AccAtomic<Acc>::add(acc_at(cur), seg, fscale);
             tau -= dtau;
             t = tn;
             const bool a0 = tm0 == tn;
@@ -1084,8 +1131,8 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
             tm1 += a1 ? td1 : 0.f;
             tm2 += (!a0 && !a1) ? td2 : 0.f;
             f = (q & W.guard) != 0u;
-            cur = q & W.offmask;
-            kap_c = kapB[cur];  // the next cell, read ahead
+            cur = q & offmask;
+            kap_c = kap_at(cur);  // the next cell, read ahead
           }
           if (f) {
             live = false;
@@ -1100,8 +1147,8 @@ __global__ void __launch_bounds__(kThreads) tiles_process_kernel(
     __syncthreads();
     // flush the visit: J deposits, tail of the output region, face counts
     {
-      double *Jt = J + (size_t)tile * G.tile_cells;
-      for (int i = threadIdx.x; i < G.tile_cells; i += kThreads) {
+      double *Jt = J + (size_t)tile * tile_cells;
+      for (int i = threadIdx.x; i < tile_cells; i += kThreads) {
         const Acc v = accL[i];
         if (v != (Acc)0) atomicAdd(&Jt[i], AccAtomic<Acc>::value(v, G.fix_inv));
       }
@@ -1200,7 +1247,7 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
   const ProbGeo &P = G.probs[0];
   const WalkLayout W = G.W;
   const int T0 = G.T[0], T1 = G.T[1], T2 = G.T[2];
-  const int st0 = T1 * T2, st1 = T2;
+  const int st0 = (T1 * T2) << 2, st1 = T2 << 2;  // byte strides
   const int PN0 = P.N[0], PN1 = P.N[1], PN2 = P.N[2];
   uint32_t my_cross = 0, my_hops = 0, my_abs = 0, my_esc = 0, my_err = 0;
   for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < n; j += gridDim.x * kThreads) {
@@ -1230,13 +1277,13 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
         dda_axis(l1, p1, d1, tm1, td1);
         dda_axis(l2, p2, d2, tm2, td2);
         const uint32_t inc0 = walk_inc(W.fs[0], st0, d0), inc1 = walk_inc(W.fs[1], st1, d1),
-                       inc2 = walk_inc(W.fs[2], 1, d2);
+                       inc2 = walk_inc(W.fs[2], 4, d2);
         uint32_t q = walk_state(W, l0, l1, l2, d0, d1, d2, T0, T1, T2);
         const uint32_t q_sum0 = walk_sum(W, q, T0, T1, T2);
-        // indexed by the biased offset of the walk state
+        // indexed by the biased cell of the walk state (its byte offset / 4)
         const float *kt = kappa + (size_t)tile * G.tile_cells - W.bias;
         double *Jt = J + (size_t)tile * G.tile_cells - W.bias;
-        float kap_c = kt[q & W.offmask];
+        float kap_c = kt[(q & W.offmask) >> 2];
         t = 0.f;
         bool exited = false;
         for (;;) {
@@ -1244,7 +1291,7 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
           const float seg = tn - t;
           const float dtau = seg * kap_c;
           if (dtau >= tau) break;
-          atomicAdd(&Jt[q & W.offmask], walkout_dep(G, seg));
+          atomicAdd(&Jt[(q & W.offmask) >> 2], walkout_dep(G, seg));
           tau -= dtau;
           t = tn;
           const bool a0 = tm0 == tn;
@@ -1257,13 +1304,13 @@ __global__ void __launch_bounds__(kThreads) tiles_walkout_kernel(
             exited = true;
             break;
           }
-          kap_c = kt[q & W.offmask];
+          kap_c = kt[(q & W.offmask) >> 2];
         }
         pc_cross += (walk_sum(W, q, T0, T1, T2) - q_sum0) + (exited ? 0u : 1u);
         if (!exited) {
           // absorbed: the partial last step, up to the target optical depth
           const float sa = kap_c > 0.f ? tau * __builtin_amdgcn_rcpf(kap_c) : 0.f;
-          atomicAdd(&Jt[q & W.offmask], walkout_dep(G, sa));
+          atomicAdd(&Jt[(q & W.offmask) >> 2], walkout_dep(G, sa));
           t = t + sa;
           status = 0;
           break;
@@ -1514,27 +1561,6 @@ constexpr size_t kLdsBytes = 160 * 1024;  // LDS of one CU on gfx950
 
 // The walk-state layout of a tile shape (WalkLayout); bits() of the result
 // is at most 32 for every tile set_tile accepts.
-static WalkLayout walk_layout(const int T[3]) {
-  WalkLayout W{};
-  const int cells = T[0] * T[1] * T[2];
-  W.bias = T[1] * T[2];
-  W.pad = (W.bias + 3) & ~3;
-  // biased offsets one step past either face: [0, cells + 2 bias)
-  uint32_t ob = 0;
-  while ((1u << ob) < (uint32_t)(cells + 2 * W.bias)) ++ob;
-  uint32_t w[3];
-  for (int a = 0; a < 3; ++a) {
-    w[a] = 1;
-    while ((1 << (w[a] - 1)) < T[a]) ++w[a];
-  }
-  W.fs[2] = ob;
-  W.fs[1] = ob + w[2];
-  W.fs[0] = ob + w[2] + w[1];
-  for (int a = 0; a < 3; ++a) W.guard_a[a] = (uint32_t)T[a] << W.fs[a];
-  W.guard = W.guard_a[0] | W.guard_a[1] | W.guard_a[2];
-  W.offmask = (1u << ob) - 1u;
-  return W;
-}
 
 static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
   TileGeo G{};
@@ -1543,7 +1569,7 @@ static TileGeo make_geo(const cmi_ctx *c, const TilesEngine *e) {
     G.tshift[a] = c->g.tshift[a];
   }
   G.tile_cells = c->g.tile_cells;
-  G.W = walk_layout(G.T);
+  G.W = walk_layout(G.T[0], G.T[1], G.T[2]);
   G.ntiles = e->ntiles;
   G.nprob = e->nprob;
   G.probs = e->d_probs;
@@ -1714,38 +1740,38 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
                            bool with_pid, PacketOut po,
                            unsigned long long *prob_cross) {
   TilesEngine *e = c->tiles;
-#define CMI_PROC(ACC, TH, LIST)                                                  \
-  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST, false>), dim3(R.grid), dim3(TH), \
-                     R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc, e->ndesc, \
-                     e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,   \
-                     LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po, \
+#define CMI_PROC(ACC, TH, LIST, RE, TL)                                            \
+  hipLaunchKernelGGL((tiles_process_kernel<ACC, TH, LIST, RE, TL>), dim3(R.grid), dim3(TH), \
+                     (TL) ? 0 : R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,      \
+                     e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist, \
+                     LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po,       \
                      c->d_cnt, prob_cross, R.refill, e->rel)
+  // 16^3 tiles at 512 threads (the tuned shape): the kernel specialised to
+  // them, with its LDS block static
+  const bool t16 = R.threads == 512 && G.T[0] == 16 && G.T[1] == 16 && G.T[2] == 16;
   if (G.reemit_p > 0.f) {
     // reemission: emission-driven transports only
-    if (R.threads == 256)
-      hipLaunchKernelGGL((tiles_process_kernel<FixAcc, 256, false, true>), dim3(R.grid),
-                         dim3(256), R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,
-                         e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,
-                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill, e->rel);
-    else
-      hipLaunchKernelGGL((tiles_process_kernel<FixAcc, 512, false, true>), dim3(R.grid),
-                         dim3(512), R.lds, c->stream, G, c->f.kappa, c->f.J, e->desc,
-                         e->ndesc, e->ticket, e->P[0], e->idx, e->P[1], e->dest[1], e->hist,
-                         nullptr, nullptr, po, c->d_cnt, prob_cross, R.refill, e->rel);
+    if (R.threads == 256) CMI_PROC(FixAcc, 256, false, true, 0);
+    else if (t16) CMI_PROC(FixAcc, 512, false, true, 16);
+    else CMI_PROC(FixAcc, 512, false, true, 0);
     return;
   }
-  if (with_pid) {
-    if (R.fp32acc) {
-      if (R.threads == 256) CMI_PROC(float, 256, true); else CMI_PROC(float, 512, true);
+  if (R.fp32acc) {
+    if (with_pid) {
+      if (R.threads == 256) CMI_PROC(float, 256, true, false, 0);
+      else CMI_PROC(float, 512, true, false, 0);
     } else {
-      if (R.threads == 256) CMI_PROC(FixAcc, 256, true); else CMI_PROC(FixAcc, 512, true);
+      if (R.threads == 256) CMI_PROC(float, 256, false, false, 0);
+      else CMI_PROC(float, 512, false, false, 0);
     }
+  } else if (with_pid) {
+    if (R.threads == 256) CMI_PROC(FixAcc, 256, true, false, 0);
+    else if (t16) CMI_PROC(FixAcc, 512, true, false, 16);
+    else CMI_PROC(FixAcc, 512, true, false, 0);
   } else {
-    if (R.fp32acc) {
-      if (R.threads == 256) CMI_PROC(float, 256, false); else CMI_PROC(float, 512, false);
-    } else {
-      if (R.threads == 256) CMI_PROC(FixAcc, 256, false); else CMI_PROC(FixAcc, 512, false);
-    }
+    if (R.threads == 256) CMI_PROC(FixAcc, 256, false, false, 0);
+    else if (t16) CMI_PROC(FixAcc, 512, false, false, 16);
+    else CMI_PROC(FixAcc, 512, false, false, 0);
   }
 #undef CMI_PROC
 }

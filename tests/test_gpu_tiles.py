@@ -245,6 +245,50 @@ def test_tiles_walkout_matches_visits():
         assert 0 < s["walkout_hops"] < s["hops"]
 
 
+def test_tiles_16_kernel_matches_any_shape_kernel():
+    """16^3 tiles at 512 threads run a process kernel specialised to that
+    shape (a static LDS block, the walk layout as constants).  The same
+    packets through it, through the kernel for any shape (256 threads) and
+    entirely through the walk-out give the same statuses, crossings, tile
+    visits and final positions, and J to the order of its FP64 sums."""
+    ncell, nsub = [64, 64, 64], [4, 4, 4]
+    box = W.box_10pc()
+    n, xn = _field(9, ncell, nsub, -6, -3)
+    pos, L = W.random_sources(4, box, seed=13)
+    N, NE = 200000, 300000
+    with cmi.Context(box, ncell, nsub, cmi.PREC_FP64) as c64:
+        c64.set_sources(pos, L)
+        pk, ss = c64.emit(N, 5, 0)
+    out = {}
+    for name, threads, fp in (("t16", 512, -1), ("any", 256, -1), ("walkout", 512, 10 ** 9)):
+        with cmi.Context(box, ncell, nsub, cmi.PREC_FP32) as ctx:
+            ctx.set_engine(cmi.ENGINE_TILES, tile=(16, 16, 16), block_threads=threads,
+                           finish_packets=fp)
+            ctx.set_fields(n, xn)
+            p, st, cr, hp, stats = ctx.transport_packets(pk, ss)
+            J, _ = ctx.get_fields()
+            ctx.set_sources(pos, L)
+            ctx.reset_intensity()
+            s2 = ctx.transport(NE, 6, 1)
+            J2, _ = ctx.get_fields()
+        out[name] = (p, st, cr, hp, stats, J, s2, J2)
+    p0, st0, cr0, hp0, s0, J0, e0, Je0 = out["t16"]
+    assert s0["errors"] == 0 and s0["absorbed"] + s0["escaped"] == N
+    assert s0["walkout_crossings"] == 0 and s0["rounds"] > 1
+    for name in ("any", "walkout"):
+        p, st, cr, hp, stats, J, s2, J2 = out[name]
+        assert np.array_equal(st, st0), name
+        assert np.array_equal(cr, cr0), name
+        assert np.array_equal(hp, hp0), name
+        assert np.array_equal(p["pos"], p0["pos"]), name
+        for k in ("crossings", "absorbed", "escaped", "hops"):
+            assert stats[k] == s0[k], (name, k)
+            assert s2[k] == e0[k], (name, k)
+        assert np.allclose(J, J0, rtol=1e-10, atol=0), name
+        assert np.allclose(J2, Je0, rtol=1e-10, atol=0), name
+    assert out["walkout"][4]["walkout_crossings"] == s0["crossings"]
+
+
 def test_tiles_results_independent_of_chunking():
     """How a round's packets are cut into workgroup visits (the chunk, which
     the plan also shrinks for rounds with few packets) changes only the order
