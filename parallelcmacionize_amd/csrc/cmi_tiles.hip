@@ -1749,12 +1749,14 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
                      LIST ? e->pid[0] : nullptr, LIST ? e->pid[1] : nullptr, po,       \
                      c->d_cnt, prob_cross, R.refill, e->rel)
   // 16^3 tiles at 512 threads (the tuned shape): the kernel specialised to
-  // them, with its LDS block static
+  // them, with its LDS block static.  Emission-driven transports without
+  // reemission only: the specialisation's constants cost the packet-list and
+  // reemitting kernels registers (89 and 99 VGPRs against 75 and 86), and
+  // with them a wave per SIMD.
   const bool t16 = R.threads == 512 && G.T[0] == 16 && G.T[1] == 16 && G.T[2] == 16;
   if (G.reemit_p > 0.f) {
     // reemission: emission-driven transports only
     if (R.threads == 256) CMI_PROC(FixAcc, 256, false, true, 0);
-    else if (t16) CMI_PROC(FixAcc, 512, false, true, 16);
     else CMI_PROC(FixAcc, 512, false, true, 0);
     return;
   }
@@ -1768,7 +1770,6 @@ static void launch_process(cmi_ctx *c, const TileGeo &G, const RoundCfg &R,
     }
   } else if (with_pid) {
     if (R.threads == 256) CMI_PROC(FixAcc, 256, true, false, 0);
-    else if (t16) CMI_PROC(FixAcc, 512, true, false, 16);
     else CMI_PROC(FixAcc, 512, true, false, 0);
   } else {
     if (R.threads == 256) CMI_PROC(FixAcc, 256, false, false, 0);

@@ -246,11 +246,13 @@ def test_tiles_walkout_matches_visits():
 
 
 def test_tiles_16_kernel_matches_any_shape_kernel():
-    """16^3 tiles at 512 threads run a process kernel specialised to that
-    shape (a static LDS block, the walk layout as constants).  The same
-    packets through it, through the kernel for any shape (256 threads) and
-    entirely through the walk-out give the same statuses, crossings, tile
-    visits and final positions, and J to the order of its FP64 sums."""
+    """Emission-driven transports of 16^3 tiles at 512 threads run a process
+    kernel specialised to that shape (a static LDS block, the walk layout as
+    constants).  The same emitted packets through it, through the kernel for
+    any shape (256 threads) and entirely through the walk-out give the same
+    crossings, tile visits and fates, and J to the order of its FP64 sums;
+    an explicit packet list (the kernel for any shape at both thread counts)
+    agrees packet by packet."""
     ncell, nsub = [64, 64, 64], [4, 4, 4]
     box = W.box_10pc()
     n, xn = _field(9, ncell, nsub, -6, -3)
