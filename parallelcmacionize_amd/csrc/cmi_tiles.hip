@@ -71,10 +71,11 @@ struct ProbGeo {
 // it stays non-negative one step past any face) in the low bits (offmask),
 // and above it one step counter per axis (log2(T_a) + 1 bits at bit fs[a]).
 // Byte offsets let a visit read the opacity at the offset itself (the
-// opacities' biased base is the start of the LDS block).  The counter of axis a starts at the cells already behind
-// the packet along its direction (l_a for a positive direction, T_a - 1 - l_a
-// for a negative one) and grows by one per step along a, so the flight leaves
-// the tile exactly when a counter reaches T_a, its top bit (guard).  A step
+// opacities' biased base is the start of the LDS block).  The counter of
+// axis a starts at the cells already behind the packet along its direction
+// (l_a for a positive direction, T_a - 1 - l_a for a negative one) and grows
+// by one per step along a, so the flight leaves the tile exactly when a
+// counter reaches T_a, its top bit (guard).  A step
 // along a adds one increment, (1 << fs[a]) plus the signed byte stride along
 // a: one select chain, one add and one mask test per crossing move the cell,
 // count the step and detect the exit.  The offset field never borrows from or
