@@ -118,11 +118,14 @@ def test_tiles_transport_matches_walk(oracle):
     assert np.median(rel) < 1e-5
 
 
-@pytest.mark.parametrize("tile", [(64, 8, 4), (4, 64, 16), (2, 4, 64)])
+@pytest.mark.parametrize("tile", [(64, 8, 4), (4, 64, 16), (2, 4, 64), (64, 16, 8),
+                                  (2, 64, 64)])
 def test_tiles_wide_tiles_match_walk(tile):
     """Tiles with a 64-cell axis (the widest step counter of the packed walk
     state) on a grid of 64^3 subgrids: the same intensity field and fates
-    as the walk engine, within the FP32 bars."""
+    as the walk engine, within the FP32 bars.  (64, 16, 8) and (2, 64, 64)
+    have the largest allowed 8192 cells; the latter, with a 4096-cell
+    plane as bias, fills all 32 bits of the walk state."""
     ncell, nsub = [128, 128, 128], [2, 2, 2]
     box = W.box_10pc()
     n, xn = _field(9, ncell, nsub, -5, -2)
@@ -243,6 +246,19 @@ def test_tiles_walkout_matches_visits():
         assert s["rounds"] == 9
         assert 0 < s["walkout_crossings"] < s["crossings"]
         assert 0 < s["walkout_hops"] < s["hops"]
+
+
+def test_tiles_larger_than_8192_cells_refused():
+    """A tile of more than 8192 cells does not fit the packed walk state (its
+    byte offset and step counters): the engine refuses it and keeps the
+    previous tile."""
+    box = W.box_10pc()
+    with cmi.Context(box, [64] * 3, [1] * 3, cmi.PREC_FP32) as ctx:
+        ctx.set_engine(cmi.ENGINE_TILES, tile=(16, 16, 16))
+        for tile in ((32, 32, 16), (64, 16, 16), (64, 64, 4)):
+            with pytest.raises(cmi.CMIError):
+                ctx.set_engine(cmi.ENGINE_TILES, tile=tile)
+        ctx.set_engine(cmi.ENGINE_TILES, tile=(32, 16, 16))  # 8192: allowed
 
 
 def test_tiles_16_kernel_matches_any_shape_kernel():
