@@ -17,7 +17,7 @@ DEFAULT = {"tile": 16, "block_threads": 512, "refill_batch": 32, "chunk": 8192}
 SPACE = {
     "tile": [8, 16],      # a 32^3 tile needs 384 KiB of LDS
     "block_threads": [256, 512],
-    "refill_batch": [8, 16, 24, 28, 32, 48, 64],
+    "refill_batch": [8, 16, 24, 28, 32, 36, 40, 48, 64],
     "chunk": [4096, 8192, 12288, 14336, 16384, 24576, 32768],
 }
 
