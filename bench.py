@@ -220,6 +220,13 @@ def warm_start_section(args, box, ncell, nsub, n_ref, pos, L):
     x_plain = prob.xn_lattice()
     warm = WS.converge_warm(prob, net, N, seed=21, tol=args.tol)
     x_warm = prob.xn_lattice()
+    err64 = None
+    if not args.no_error:
+        # one more iteration from the warm-started converged field, FP32
+        # engine against the all-FP64 walk on the same packets
+        err64 = error_vs_fp64(box, ncell, nsub, n_ref,
+                              W.global_to_subgrid_major(x_warm, nsub), pos, L,
+                              21, 0, prob.ctx)
     r_ref = W.ionized_radius(x_ref, box, [ncell] * 3)
     r_plain = W.ionized_radius(x_plain, box, [ncell] * 3)
     r_warm = W.ionized_radius(x_warm, box, [ncell] * 3)
@@ -256,6 +263,7 @@ def warm_start_section(args, box, ncell, nsub, n_ref, pos, L):
                                "starts on one packet set, not against the "
                                "physical solution"),
         "network": os.path.relpath(args.net, ROOT),
+        "error_vs_fp64_at_warm_result": err64,
     }
 
 
@@ -308,6 +316,43 @@ def precision_section(args):
     return out
 
 
+def batched_error_vs_fp64(ps, spec, fields, seed=11, iteration=0):
+    """Problem 0 of a set draws the packets a single context draws with the
+    same seed and iteration: its x_n after one set iteration (FP32 tiles)
+    against one iteration of the all-FP64 GPU walk on those packets, both
+    from x_n = 1e-6."""
+    import parallelcmacionize_amd as cmi
+    from parallelcmacionize_amd import workloads as W
+    ps.restore()
+    r = ps.iteration(seed=seed, iteration=iteration)
+    x32 = ps.batch.get_fields(0)[1].reshape(-1)
+    n_lat, pos, L = fields[0]
+    e = spec[0]["n"]
+    nsub = [e // 16] * 3
+    npk = int(ps.npackets[0])
+    with cmi.Context(spec[0]["box"], [e] * 3, nsub, cmi.PREC_FP64) as c64:
+        c64.set_fields(W.global_to_subgrid_major(np.asarray(n_lat).reshape(e, e, e), nsub),
+                       np.full(e ** 3, W.XN0))
+        c64.set_sources(pos, L)
+        s64 = c64.transport(npk, seed, iteration)
+        c64.update_ionization(c64.total_luminosity(), npk, ps.alpha, form=1)
+        x64 = W.subgrid_major_to_global(c64.get_fields()[1], [e] * 3, nsub).reshape(-1)
+    ps.restore()
+    d = np.abs(x32 - x64)
+    out = {
+        "reference": ("all-FP64 GPU walk on problem 0's packets (%d, seed %d, "
+                      "iteration %d), one iteration from x_n = 1e-6"
+                      % (npk, seed, iteration)),
+        "problem0_edge": e,
+        "crossings_fp32": int(r["problem_crossings"][0]),
+        "crossings_fp64": s64["crossings"],
+        "xn_abs_mean": float(d.mean()), "xn_abs_max": float(d.max()),
+    }
+    log("batched error vs FP64 (problem 0, %d^3): x_n mean %.2e max %.2e"
+        % (e, out["xn_abs_mean"], out["xn_abs_max"]))
+    return out
+
+
 def batched_section(args):
     """64 problems of uneven size in one graph-captured problem-set
     iteration: throughput and per-problem completion times."""
@@ -323,6 +368,8 @@ def batched_section(args):
                       refill_batch=cfg["refill_batch"], chunk=cfg["chunk"])
     B.run(ps, 2, seed=1, it0=0)                     # warm-up, graph capture
     r = B.run(ps, args.batched_iterations, seed=1, it0=2)
+    if not args.no_error:
+        r["error_vs_fp64_problem0"] = batched_error_vs_fp64(ps, spec, fields)
     ps.close()
     r["tuning"] = cfg
     r["tuning_source"] = ("autotuner cache" if "batched64" in cache
